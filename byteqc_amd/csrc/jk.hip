@@ -1,0 +1,191 @@
+// Class dispatch and C-ABI entry points of the J/K build.
+//
+// Canonical classes have LA >= LB, LC >= LD and pair-id(LA,LB) >=
+// pair-id(LC,LD) with pair-id(a,b) = a(a+1)/2 + b; a task whose ket class is
+// the larger one is evaluated with bra and ket exchanged.  Kernel
+// instantiations live in jk_bra_XY.hip, one translation unit per bra class.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "jk_kernel.h"
+
+static int pair_id(int a, int b) { return a * (a + 1) / 2 + b; }
+
+static JKKernelTable g_kernels;
+static bool g_registered = false;
+
+#define BQC_DECL_BRA(LA, LB) void bqc_register_bra_##LA##LB(JKKernelTable &tab);
+BQC_DECL_BRA(0, 0)
+BQC_DECL_BRA(1, 0)
+BQC_DECL_BRA(1, 1)
+BQC_DECL_BRA(2, 0)
+BQC_DECL_BRA(2, 1)
+BQC_DECL_BRA(2, 2)
+
+static void register_kernels() {
+    if (g_registered) return;
+    bqc_register_bra_00(g_kernels);
+    bqc_register_bra_10(g_kernels);
+    bqc_register_bra_11(g_kernels);
+    bqc_register_bra_20(g_kernels);
+    bqc_register_bra_21(g_kernels);
+    bqc_register_bra_22(g_kernels);
+    g_registered = true;
+}
+
+static double g_last_flops[2] = {0, 0};
+
+// Algorithmic FLOP model of one contracted quartet (K primitive quartets).
+static double quartet_flops(int li, int lj, int lk, int ll, int K) {
+    int nr = (li + lj + lk + ll) / 2 + 1;
+    int nf = ncart(li) * ncart(lj) * ncart(lk) * ncart(ll);
+    int nmax = li + lj, mmax = lk + ll;
+    int ijmin = li < lj ? li : lj, klmin = lk < ll ? lk : ll;
+    double f2d = 4.0 * nmax + 4.0 * mmax + 6.0 * nmax * mmax;
+    for (int j = 1; j <= ijmin; ++j) f2d += 2.0 * (mmax + 1) * (nmax - j + 1);
+    for (int l = 1; l <= klmin; ++l) f2d += 2.0 * (li + 1) * (lj + 1) * (mmax - l + 1);
+    return K * (20.0 + nr * (60.0 + 20.0 + 3.0 * f2d + 3.0 * nf)) + 12.0 * nf;
+}
+
+static int launch_task(BasisProdCache *bp, const JKArgs &jk, const PairData &pd,
+                       const int *bins_ij, const int *bins_kl, int nbins, int cp_ij,
+                       int cp_kl, hipStream_t stream, double *flops) {
+    const ContractionProdType *tij = bp->cptype + cp_ij, *tkl = bp->cptype + cp_kl;
+    int li = tij->l_bra, lj = tij->l_ket, lk = tkl->l_bra, ll = tkl->l_ket;
+    if ((li + lj + lk + ll) / 2 + 1 >= 12) return 2;
+    if (li > BQC_LMAX || lj > BQC_LMAX || lk > BQC_LMAX || ll > BQC_LMAX || lj > li ||
+        ll > lk)
+        return 1;
+    if (nbins > BQC_MAX_BINS) return 3;
+    register_kernels();
+    TaskArgs ta;
+    memset(&ta, 0, sizeof(ta));
+    int pij = pair_id(li, lj), pkl = pair_id(lk, ll);
+    ta.swap = pij < pkl;
+    ta.diag = cp_ij == cp_kl;
+    const int *locs = bp->bas_pairs_locs, *plocs = bp->primitive_pairs_locs;
+    int cp_p = ta.swap ? cp_kl : cp_ij, cp_q = ta.swap ? cp_ij : cp_kl;
+    ta.bas_p0 = locs[cp_p];
+    ta.bas_q0 = locs[cp_q];
+    ta.prim_p0 = plocs[cp_p];
+    ta.prim_q0 = plocs[cp_q];
+    ta.nprim_p = bp->cptype[cp_p].nprim_12;
+    ta.nprim_q = bp->cptype[cp_q].nprim_12;
+    // work rectangles: kl pairs of bin b x ij pairs of bins [0, nbins-b)
+    int nb = 0;
+    ta.pre[0] = 0;
+    double nq = 0;
+    for (int b = 0; b < nbins; ++b) {
+        int nkl = bins_kl[b + 1] - bins_kl[b];
+        int nij = bins_ij[nbins - b] - bins_ij[0];
+        if (nkl <= 0 || nij <= 0) continue;
+        ta.klb[nb] = bins_kl[b];
+        ta.nij[nb] = nij;
+        ta.pre[nb + 1] = ta.pre[nb] + (long long)nkl * nij;
+        if (ta.diag) {
+            for (int kl = bins_kl[b]; kl < bins_kl[b + 1]; ++kl) nq += nij > kl ? nij - kl : 0;
+        } else {
+            nq += (double)nkl * nij;
+        }
+        ++nb;
+    }
+    ta.nbins = nb;
+    if (nb == 0) return 0;
+    if (bins_ij[0] != 0) return 1;
+    if (flops)
+        *flops += nq * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
+    JKKernel kern = ta.swap ? g_kernels[pkl][pij] : g_kernels[pij][pkl];
+    if (!kern) return 1;
+    long long ntot = ta.pre[nb];
+    long long nblk = (ntot + BQC_THREADS - 1) / BQC_THREADS;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, ta);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", li, lj, lk, ll,
+                hipGetErrorString(err));
+        return 1;
+    }
+    return 0;
+}
+
+static PairData make_pair_data(const BasisProdCache *bp) {
+    PairData pd;
+    pd.a12 = bp->a12;
+    pd.e12 = bp->e12;
+    pd.x12 = bp->x12;
+    pd.y12 = bp->y12;
+    pd.z12 = bp->z12;
+    pd.bas_x = bp->bas_coords;
+    pd.bas_y = bp->bas_coords + bp->nbas;
+    pd.bas_z = bp->bas_coords + 2 * bp->nbas;
+    pd.bra = bp->bas_pair2bra;
+    pd.ket = bp->bas_pair2ket;
+    pd.ao_loc = bp->ao_loc;
+    return pd;
+}
+
+extern "C" {
+
+int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
+                 int n_dm, int *bins_locs_ij, int *bins_locs_kl, int nbins, int cp_ij_id,
+                 int cp_kl_id, hipStream_t stream) {
+    JKArgs jk = {vj, vk, dm, nao, n_dm};
+    PairData pd = make_pair_data(bpcache);
+    return launch_task(bpcache, jk, pd, bins_locs_ij, bins_locs_kl, nbins, cp_ij_id,
+                       cp_kl_id, stream, nullptr);
+}
+
+int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
+                int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
+                int precision, const double *log_q, const double *task_dmax,
+                double fp32_cut, hipStream_t stream) {
+    (void)log_q;
+    (void)task_dmax;
+    (void)fp32_cut;
+    if (precision != 0) return 1;
+    JKArgs jk = {vj, vk, dm, nao, n_dm};
+    PairData pd = make_pair_data(bpcache);
+    double flops = 0;
+    for (int t = 0; t < ntask; ++t) {
+        const int *bi = task_bins + (size_t)t * 2 * (nbins + 1);
+        const int *bk = bi + nbins + 1;
+        int err = launch_task(bpcache, jk, pd, bi, bk, nbins, task_cp[2 * t],
+                              task_cp[2 * t + 1], stream, &flops);
+        if (err) return err;
+    }
+    g_last_flops[0] = flops;
+    g_last_flops[1] = 0;
+    return 0;
+}
+
+void BQClast_build_flops(double *flops) {
+    flops[0] = g_last_flops[0];
+    flops[1] = g_last_flops[1];
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// J = 2 (V + V^T), K = V + V^T
+// ---------------------------------------------------------------------------
+__global__ void symmetrize_kernel(double *v, int nao, double fac) {
+    const size_t nao2 = (size_t)nao * nao;
+    double *m = v + blockIdx.z * nao2;
+    const int i = blockIdx.y * 16 + threadIdx.y;
+    const int j = blockIdx.x * 16 + threadIdx.x;
+    if (i >= nao || j > i) return;
+    const double s = fac * (m[(size_t)i * nao + j] + m[(size_t)j * nao + i]);
+    m[(size_t)i * nao + j] = s;
+    m[(size_t)j * nao + i] = s;
+}
+
+extern "C" int BQCsymmetrize_jk(double *vj, double *vk, int nao, int n_dm,
+                                hipStream_t stream) {
+    dim3 thr(16, 16);
+    dim3 grd((nao + 15) / 16, (nao + 15) / 16, n_dm);
+    if (vj) hipLaunchKernelGGL(symmetrize_kernel, grd, thr, 0, stream, vj, nao, 2.0);
+    if (vk) hipLaunchKernelGGL(symmetrize_kernel, grd, thr, 0, stream, vk, nao, 1.0);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}

@@ -1,0 +1,284 @@
+#pragma once
+// Rys-quadrature ERI + J/K digestion kernels for gfx950 (kernel templates).
+//
+// Work decomposition.  One launch per task (pair type cp_ij with pair type
+// cp_kl, cp_ij >= cp_kl), as in the reference's per-task call
+// (cuobc/lib/gvhf/nr_jk_driver.cu:227-283), but with all Schwarz bins of the
+// task folded into one grid: kl pairs of bin b meet the ij pairs of bins
+// [0, nbins-b); the work items of bin b form a dense (kl x ij) rectangle and
+// the rectangles are laid end to end (prefix array in the task descriptor).
+// One lane evaluates one contracted shell quartet: it loops over the
+// primitive pairs of both sides, evaluates the Rys roots once per primitive
+// quartet and accumulates the Cartesian integral block in registers, then
+// digests the block into J and K with FP64 atomics.  ij is the fastest work
+// index, so a wavefront shares its kl pair (and its density block) for up to
+// 64 lanes.
+//
+// Unique-quartet weights follow the reference: e12 of a pair with ish == jsh
+// carries 1/2 (pair data), and a quartet with bas_ij == bas_kl carries 1/2
+// (cuobc/lib/gvhf/g2e.cu:1022-1028).  Digestion (contract_jk.cu:28-316):
+//   J: V[i,j] += g D[k,l], V[k,l] += g D[i,j]
+//   K: V[i,k] += g D[j,l], V[i,l] += g D[j,k], V[j,k] += g D[i,l], V[j,l] += g D[i,k]
+// followed by J = 2 (V + V^T), K = V + V^T (BQCsymmetrize_jk).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "../../include/byteqc_jk.h"
+#include "bqc_device.h"
+
+using namespace bqc;
+
+#define BQC_MAX_BINS 64
+#define BQC_THREADS 256
+
+struct TaskArgs {
+    int bas_p0, bas_q0;    // first global pair index of the p / q side
+    int prim_p0, prim_q0;  // first primitive pair of the p / q side
+    int nprim_p, nprim_q;  // primitive pairs per shell pair
+    int diag;              // cp_ij == cp_kl
+    int swap;              // p side is the task's kl list
+    int nbins;
+    int klb[BQC_MAX_BINS + 1];
+    int nij[BQC_MAX_BINS];
+    long long pre[BQC_MAX_BINS + 1];
+};
+
+struct JKArgs {
+    double *vj;
+    double *vk;
+    const double *dm;
+    int nao;
+    int n_dm;
+};
+
+struct PairData {
+    const double *a12, *e12, *x12, *y12, *z12;
+    const double *bas_x, *bas_y, *bas_z;
+    const int *bra, *ket, *ao_loc;
+};
+
+__device__ __forceinline__ void atomic_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Integral block of (p | q): p = shells (a, b) of the "bra" pair, q = (c, d).
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void eri_block(double *__restrict__ gout, const PairData &pd,
+                                          int pp0, int np_p, int qq0, int np_q, double norm,
+                                          const double *ra, const double *rb,
+                                          const double *rc, const double *rd) {
+    using C = Cls<LA, LB, LC, LD>;
+    constexpr int NR = C::NR;
+#pragma unroll
+    for (int n = 0; n < C::NF; ++n) gout[n] = 0.0;
+    const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+    const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
+    for (int ip = 0; ip < np_p; ++ip) {
+        const int p = pp0 + ip;
+        const double aij = pd.a12[p], eij = pd.e12[p];
+        const double P[3] = {pd.x12[p], pd.y12[p], pd.z12[p]};
+        for (int iq = 0; iq < np_q; ++iq) {
+            const int q = qq0 + iq;
+            const double akl = pd.a12[q], ekl = pd.e12[q];
+            const double Q[3] = {pd.x12[q], pd.y12[q], pd.z12[q]};
+            const double pq[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+            const double aijkl = aij + akl;
+            const double a1 = aij * akl;
+            const double a0 = a1 / aijkl;
+            const double x = a0 * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]);
+            double u[NR], w[NR];
+            rys_roots<NR>(x, u, w);
+            const double fac = eij * ekl / (sqrt(aijkl) * a1);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double u2 = a0 * u[r];
+                const double tmp4 = 0.5 / (u2 * aijkl + a1);
+                const double b00 = u2 * tmp4;
+                const double b10 = b00 + tmp4 * akl;
+                const double b01 = b00 + tmp4 * aij;
+                const double tmp2 = 2.0 * b00 * akl;
+                const double tmp3 = 2.0 * b00 * aij;
+                double Ix[C::GSZ], Iy[C::GSZ], Iz[C::GSZ];
+                rys2d<LA, LB, LC, LD, double>(Ix, norm, (P[0] - ra[0]) - tmp2 * pq[0],
+                                              (Q[0] - rc[0]) + tmp3 * pq[0], b10, b01, b00,
+                                              ab[0], cd[0]);
+                rys2d<LA, LB, LC, LD, double>(Iy, fac, (P[1] - ra[1]) - tmp2 * pq[1],
+                                              (Q[1] - rc[1]) + tmp3 * pq[1], b10, b01, b00,
+                                              ab[1], cd[1]);
+                rys2d<LA, LB, LC, LD, double>(Iz, w[r], (P[2] - ra[2]) - tmp2 * pq[2],
+                                              (Q[2] - rc[2]) + tmp3 * pq[2], b10, b01, b00,
+                                              ab[2], cd[2]);
+#pragma unroll
+                for (int n = 0; n < C::NF; ++n) {
+                    const int ci = n / (C::NFJ * C::NFK * C::NFL);
+                    const int cj = (n / (C::NFK * C::NFL)) % C::NFJ;
+                    const int ck = (n / C::NFL) % C::NFK;
+                    const int cl = n % C::NFL;
+                    gout[n] += Ix[C::g(cart_x(LA, ci), cart_x(LB, cj), cart_x(LC, ck),
+                                       cart_x(LD, cl))] *
+                               Iy[C::g(cart_y(LA, ci), cart_y(LB, cj), cart_y(LC, ck),
+                                       cart_y(LD, cl))] *
+                               Iz[C::g(cart_z(LA, ci), cart_z(LB, cj), cart_z(LC, ck),
+                                       cart_z(LD, cl))];
+                }
+            }
+        }
+    }
+}
+
+// J/K digestion of one integral block (a b | c d) with AO offsets a0..d0.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void digest(const double *__restrict__ gout, const JKArgs &jk,
+                                       int a0, int b0, int c0, int d0) {
+    using C = Cls<LA, LB, LC, LD>;
+    const int nao = jk.nao;
+    const size_t nao2 = (size_t)nao * nao;
+    for (int idm = 0; idm < jk.n_dm; ++idm) {
+        const double *__restrict__ D = jk.dm + idm * nao2;
+        if (jk.vj) {
+            double *V = jk.vj + idm * nao2;
+#pragma unroll
+            for (int ia = 0; ia < C::NFI; ++ia)
+#pragma unroll
+                for (int ib = 0; ib < C::NFJ; ++ib) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ic = 0; ic < C::NFK; ++ic)
+#pragma unroll
+                        for (int id = 0; id < C::NFL; ++id)
+                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                                 D[(c0 + ic) * nao + d0 + id];
+                    atomic_add(V + (size_t)(a0 + ia) * nao + b0 + ib, s);
+                }
+#pragma unroll
+            for (int ic = 0; ic < C::NFK; ++ic)
+#pragma unroll
+                for (int id = 0; id < C::NFL; ++id) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ia = 0; ia < C::NFI; ++ia)
+#pragma unroll
+                        for (int ib = 0; ib < C::NFJ; ++ib)
+                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                                 D[(a0 + ia) * nao + b0 + ib];
+                    atomic_add(V + (size_t)(c0 + ic) * nao + d0 + id, s);
+                }
+        }
+        if (jk.vk) {
+            double *V = jk.vk + idm * nao2;
+            // V[a,c] += g D[b,d]; V[a,d] += g D[b,c]
+#pragma unroll
+            for (int ia = 0; ia < C::NFI; ++ia) {
+#pragma unroll
+                for (int ic = 0; ic < C::NFK; ++ic) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ib = 0; ib < C::NFJ; ++ib)
+#pragma unroll
+                        for (int id = 0; id < C::NFL; ++id)
+                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                                 D[(b0 + ib) * nao + d0 + id];
+                    atomic_add(V + (size_t)(a0 + ia) * nao + c0 + ic, s);
+                }
+#pragma unroll
+                for (int id = 0; id < C::NFL; ++id) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ib = 0; ib < C::NFJ; ++ib)
+#pragma unroll
+                        for (int ic = 0; ic < C::NFK; ++ic)
+                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                                 D[(b0 + ib) * nao + c0 + ic];
+                    atomic_add(V + (size_t)(a0 + ia) * nao + d0 + id, s);
+                }
+            }
+            // V[b,c] += g D[a,d]; V[b,d] += g D[a,c]
+#pragma unroll
+            for (int ib = 0; ib < C::NFJ; ++ib) {
+#pragma unroll
+                for (int ic = 0; ic < C::NFK; ++ic) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ia = 0; ia < C::NFI; ++ia)
+#pragma unroll
+                        for (int id = 0; id < C::NFL; ++id)
+                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                                 D[(a0 + ia) * nao + d0 + id];
+                    atomic_add(V + (size_t)(b0 + ib) * nao + c0 + ic, s);
+                }
+#pragma unroll
+                for (int id = 0; id < C::NFL; ++id) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ia = 0; ia < C::NFI; ++ia)
+#pragma unroll
+                        for (int ic = 0; ic < C::NFK; ++ic)
+                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                                 D[(a0 + ia) * nao + c0 + ic];
+                    atomic_add(V + (size_t)(b0 + ib) * nao + d0 + id, s);
+                }
+            }
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(BQC_THREADS)
+    jk_quartets(JKArgs jk, PairData pd, TaskArgs ta) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ta.pre[ta.nbins]) return;
+    int b = 0;
+    while (t >= ta.pre[b + 1]) ++b;
+    const long long local = t - ta.pre[b];
+    const int kl = ta.klb[b] + (int)(local / ta.nij[b]);
+    const int ij = (int)(local % ta.nij[b]);
+    // p side = bra pair as evaluated; the task's (ij, kl) roles swap when the
+    // ket class is the larger one
+    const int bas_ij = (ta.swap ? ta.bas_q0 : ta.bas_p0) + ij;
+    const int bas_kl = (ta.swap ? ta.bas_p0 : ta.bas_q0) + kl;
+    if (ta.diag && bas_ij < bas_kl) return;
+    double norm = TWO_PI_52;
+    if (bas_ij == bas_kl) norm *= 0.5;
+    const int bp = ta.swap ? bas_kl : bas_ij;
+    const int bq = ta.swap ? bas_ij : bas_kl;
+    const int pp0 = ta.prim_p0 + (bp - ta.bas_p0) * ta.nprim_p;
+    const int qq0 = ta.prim_q0 + (bq - ta.bas_q0) * ta.nprim_q;
+    const int sa = pd.bra[bp], sb = pd.ket[bp], sc = pd.bra[bq], sd = pd.ket[bq];
+    const double ra[3] = {pd.bas_x[sa], pd.bas_y[sa], pd.bas_z[sa]};
+    const double rb[3] = {pd.bas_x[sb], pd.bas_y[sb], pd.bas_z[sb]};
+    const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
+    const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
+    double gout[Cls<LA, LB, LC, LD>::NF];
+    eri_block<LA, LB, LC, LD>(gout, pd, pp0, ta.nprim_p, qq0, ta.nprim_q, norm, ra, rb, rc, rd);
+    digest<LA, LB, LC, LD>(gout, jk, pd.ao_loc[sa], pd.ao_loc[sb], pd.ao_loc[sc],
+                           pd.ao_loc[sd]);
+}
+
+
+typedef void (*JKKernel)(JKArgs, PairData, TaskArgs);
+#define BQC_LMAX 3
+#define BQC_NPAIRID ((BQC_LMAX + 1) * (BQC_LMAX + 2) / 2)
+typedef JKKernel JKKernelTable[BQC_NPAIRID][BQC_NPAIRID];
+
+template <int LA, int LB, int LC, int LD>
+static void reg_kernel(JKKernelTable &tab) {
+    tab[LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_quartets<LA, LB, LC, LD>;
+}
+
+// all ket classes (LC LD) with pair-id(LC,LD) <= pair-id(LA,LB)
+template <int LA, int LB>
+static void reg_bra(JKKernelTable &tab) {
+    constexpr int pab = LA * (LA + 1) / 2 + LB;
+    reg_kernel<LA, LB, 0, 0>(tab);
+    if constexpr (pab >= 1) reg_kernel<LA, LB, 1, 0>(tab);
+    if constexpr (pab >= 2) reg_kernel<LA, LB, 1, 1>(tab);
+    if constexpr (pab >= 3) reg_kernel<LA, LB, 2, 0>(tab);
+    if constexpr (pab >= 4) reg_kernel<LA, LB, 2, 1>(tab);
+    if constexpr (pab >= 5) reg_kernel<LA, LB, 2, 2>(tab);
+    if constexpr (pab >= 6) reg_kernel<LA, LB, 3, 0>(tab);
+    if constexpr (pab >= 7) reg_kernel<LA, LB, 3, 1>(tab);
+    if constexpr (pab >= 8) reg_kernel<LA, LB, 3, 2>(tab);
+    if constexpr (pab >= 9) reg_kernel<LA, LB, 3, 3>(tab);
+}

@@ -1,0 +1,132 @@
+/*
+ * C-ABI of libbqcjk.so -- MI355X (gfx950) direct-SCF J/K engine.
+ *
+ * The first block of entry points keeps the names, argument meaning and
+ * return codes of the reference's native interface for this path, so that
+ * the reference's host code (cuobc/scf/hf.py, which binds them through
+ * ctypes) can load this library in place of libgint.so / libgvhf.so.
+ * The second block holds entry points of this library that have no
+ * counterpart there.
+ *
+ * Conventions shared by all entry points:
+ *   - plain pointers and sizes; "device" pointers are HIP device memory,
+ *     "host" pointers are ordinary host memory;
+ *   - all J/K output is accumulated (+=); the caller zeroes it;
+ *   - work is queued on the given HIP stream and returns before it ends;
+ *   - return 0 on success, non-zero on error (see each entry).
+ */
+#ifndef BYTEQC_JK_H
+#define BYTEQC_JK_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* (l_bra, l_ket, nprim_bra*nprim_ket, number of pairs) of one pair type.
+ * Same layout as the reference (cuobc/lib/gint/gint.h:110-116). */
+typedef struct {
+    int l_bra;
+    int l_ket;
+    int nprim_12;
+    int npairs;
+} ContractionProdType;
+
+/* Shell-pair cache.  Same field order and meaning as the reference
+ * (cuobc/lib/gint/gint.h:127-147; ctypes mirror cuobc/scf/hf.py:460-478):
+ * the first seven fields are host data, the rest device data once the
+ * struct has been cloned by GINTinit_basis_prod_gpu. */
+typedef struct {
+    int nbas;
+    int ncptype;
+    ContractionProdType *cptype;
+    int *bas_pairs_locs;
+    int *primitive_pairs_locs;
+    int *bas_pair2shls;
+    double *aexyz;
+    double *bas_coords;
+    int *bas_pair2bra;
+    int *bas_pair2ket;
+    int *ao_loc;
+    double *a12;
+    double *e12;
+    double *x12;
+    double *y12;
+    double *z12;
+} BasisProdCache;
+
+/* ---- entry points with a reference counterpart ------------------------- */
+
+/* Replaces GINTinit_basis_prod_cpu (cuobc/lib/gint/nr_fill_ao_ints.cu:452-470).
+ * Allocates *pbp (host) with the pair types, primitive-pair offsets and the
+ * primitive-pair table aexyz = [a12 | e12 | x12 | y12 | z12]; e12 of pairs
+ * with ish == jsh is scaled by diag_fac.  Keeps (aliases) the caller's
+ * bas_pair2shls (2 x npairs, bra row then ket row) and bas_pairs_locs
+ * (ncptype+1), which must outlive the cache.  Fills bas_coords (3 x nbas,
+ * x row, y row, z row) from atm/env. */
+void GINTinit_basis_prod_cpu(BasisProdCache **pbp, double diag_fac, double *bas_coords,
+                             int *bas_pair2shls, int *bas_pairs_locs, int ncptype,
+                             int *atm, int natm, int *bas, int nbas, double *env);
+
+/* Replaces GINTinit_basis_prod_gpu (cuobc/lib/gint/nr_fill_ao_ints.cu:472-491).
+ * *out becomes a host-side clone of *in whose device fields point into the
+ * caller-owned device arrays ao_loc, bas_coords, aexyz, bas_pair2shls. */
+void GINTinit_basis_prod_gpu(BasisProdCache **in, BasisProdCache **out, int *ao_loc,
+                             double *bas_coords, double *aexyz, int *bas_pair2shls);
+
+/* Replaces GINTdel_basis_prod (cuobc/lib/gint/nr_fill_ao_ints.cu:394-410).
+ * Frees what GINTinit_basis_prod_cpu allocated (the aliased caller arrays
+ * are not freed).  Clones made by GINTinit_basis_prod_gpu are freed with
+ * BQCdel_basis_prod_clone. */
+void GINTdel_basis_prod(BasisProdCache **pbp);
+
+/* Replaces GINTbuild_jk (cuobc/lib/gvhf/nr_jk_driver.cu:227-283).
+ * One task: pair type cp_ij_id with pair type cp_kl_id (cp_ij_id >= cp_kl_id).
+ * bins_locs_ij / bins_locs_kl (host, nbins+1) split each pair list into
+ * Schwarz bins; kl pairs of bin b meet ij pairs of bins [0, nbins-b).
+ * vj / vk / dm: device, n_dm x nao x nao (vj or vk may be NULL).
+ * Returns 0 ok, 1 launch error or unsupported class, 2 nroots >= 12. */
+int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
+                 int n_dm, int *bins_locs_ij, int *bins_locs_kl, int nbins, int cp_ij_id,
+                 int cp_kl_id, hipStream_t stream);
+
+/* ---- entry points of this library only ---------------------------------- */
+
+/* Frees a clone made by GINTinit_basis_prod_gpu (the reference leaks it). */
+void BQCdel_basis_prod_clone(BasisProdCache **pbp);
+
+/* All tasks of one J/K build in one call.  task_cp (host, 2 x ntask):
+ * (cp_ij, cp_kl) per task; task_bins (host, ntask x 2 x (nbins+1)): ij bins
+ * then kl bins per task, as for GINTbuild_jk.  precision: 0 = FP64
+ * throughout; 1 = mixed (quartets whose bound q_ij q_kl dmax falls below
+ * fp32_cut are evaluated in FP32 after FP64 Rys roots; accumulation FP64).
+ * q_ij for the mixed rule comes from log_q (device, one per pair, same order
+ * as the pair lists) and dmax is the task's density bound (task_dmax, host).
+ * Returns 0 ok, 1 launch error / unsupported class, 2 nroots >= 12,
+ * 3 too many bins. */
+int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
+                int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
+                int precision, const double *log_q, const double *task_dmax,
+                double fp32_cut, hipStream_t stream);
+
+/* Schwarz bounds on the GPU: q[i*nbas + j] = sqrt(max |(ij|ij)|) over the
+ * Cartesian components of shells i, j (1e-100 when zero), for the shell
+ * groups [grp_offsets[g], grp_offsets[g+1]) (ngroups groups, host array);
+ * atm/bas/env are device copies of the integral tables.  Returns 0 ok. */
+int BQCq_cond(double *q, const int *atm, const int *bas, const double *env, int nbas,
+              const int *grp_offsets, int ngroups, const int *grp_l, hipStream_t stream);
+
+/* J = 2 (V + V^T), K = V + V^T in place on device n_dm x nao x nao arrays
+ * (either may be NULL). */
+int BQCsymmetrize_jk(double *vj, double *vk, int nao, int n_dm, hipStream_t stream);
+
+/* Number of FP64 (and FP32) floating-point operations of the last
+ * BQCbuild_jk call's quartet list by the algorithmic model of the path
+ * (host counters, written into flops[0..1]). */
+void BQClast_build_flops(double *flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
