@@ -7,16 +7,19 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <vector>
 
-#include "jk_kernel.h"
+#include "jk_v2.h"
 
 static int pair_id(int a, int b) { return a * (a + 1) / 2 + b; }
 
 static JKKernelTable g_kernels;
+static RowKernelTable g_rows;
 static bool g_registered = false;
 
-#define BQC_DECL_BRA(LA, LB) void bqc_register_bra_##LA##LB(JKKernelTable &tab);
+#define BQC_DECL_BRA(LA, LB) \
+    void bqc_register_bra_##LA##LB(JKKernelTable &t1, RowKernelTable &t2);
 BQC_DECL_BRA(0, 0)
 BQC_DECL_BRA(1, 0)
 BQC_DECL_BRA(1, 1)
@@ -26,12 +29,12 @@ BQC_DECL_BRA(2, 2)
 
 static void register_kernels() {
     if (g_registered) return;
-    bqc_register_bra_00(g_kernels);
-    bqc_register_bra_10(g_kernels);
-    bqc_register_bra_11(g_kernels);
-    bqc_register_bra_20(g_kernels);
-    bqc_register_bra_21(g_kernels);
-    bqc_register_bra_22(g_kernels);
+    bqc_register_bra_00(g_kernels, g_rows);
+    bqc_register_bra_10(g_kernels, g_rows);
+    bqc_register_bra_11(g_kernels, g_rows);
+    bqc_register_bra_20(g_kernels, g_rows);
+    bqc_register_bra_21(g_kernels, g_rows);
+    bqc_register_bra_22(g_kernels, g_rows);
     g_registered = true;
 }
 
@@ -126,6 +129,146 @@ static PairData make_pair_data(const BasisProdCache *bp) {
     return pd;
 }
 
+// ---------------------------------------------------------------------------
+// wavefront-stationary kernels: host-side list order and work items
+// ---------------------------------------------------------------------------
+struct RowLaunch {
+    RowKernel kern;
+    V2Task task;
+    size_t off_perm_p, off_perm_q, off_items;
+};
+
+static int *g_dws = nullptr;         // device workspace (ints)
+static size_t g_dws_cap = 0;
+static int *g_hstage = nullptr;      // pinned staging
+static size_t g_hstage_cap = 0;
+static hipEvent_t g_stage_evt = nullptr;
+static bool g_stage_pending = false;
+
+static int stage_to_device(const std::vector<int> &stage, hipStream_t stream) {
+    size_t n = stage.size();
+    if (g_stage_pending) {
+        hipEventSynchronize(g_stage_evt);
+        g_stage_pending = false;
+    }
+    if (n > g_hstage_cap) {
+        if (g_hstage) hipHostFree(g_hstage);
+        size_t cap = n + n / 2 + 1024;
+        if (hipHostMalloc((void **)&g_hstage, cap * sizeof(int), hipHostMallocDefault) != hipSuccess)
+            return 1;
+        g_hstage_cap = cap;
+    }
+    if (n > g_dws_cap) {
+        // the previous workspace may still be read by queued kernels
+        hipStreamSynchronize(stream);
+        if (g_dws) hipFree(g_dws);
+        size_t cap = n + n / 2 + 1024;
+        if (hipMalloc((void **)&g_dws, cap * sizeof(int)) != hipSuccess) return 1;
+        g_dws_cap = cap;
+    }
+    if (!g_stage_evt) hipEventCreateWithFlags(&g_stage_evt, hipEventDisableTiming);
+    memcpy(g_hstage, stage.data(), n * sizeof(int));
+    if (hipMemcpyAsync(g_dws, g_hstage, n * sizeof(int), hipMemcpyHostToDevice, stream) !=
+        hipSuccess)
+        return 1;
+    hipEventRecord(g_stage_evt, stream);
+    g_stage_pending = true;
+    return 0;
+}
+
+// List order of one side: plan positions [0, bins[nbins]) grouped by bin,
+// inside a bin by first shell (bra_order gives the plan positions of the
+// pair type sorted by first shell).
+static void side_order(const int *bins, int nbins, const int *order, int npairs_cp,
+                       std::vector<int> &stage) {
+    const int n = bins[nbins];
+    std::vector<int> cursor(bins, bins + nbins);
+    size_t base = stage.size();
+    stage.resize(base + n);
+    for (int k = 0; k < npairs_cp; ++k) {
+        int pos = order[k];
+        if (pos >= n) continue;
+        int b = (int)(std::upper_bound(bins + 1, bins + nbins + 1, pos) - (bins + 1));
+        stage[base + cursor[b]++] = pos;
+    }
+}
+
+// returns -1 if the task's class has no row kernel (caller falls back)
+static int plan_rows(BasisProdCache *bp, const int *bins_ij, const int *bins_kl, int nbins,
+                     int cp_ij, int cp_kl, const int *bra_order, std::vector<int> &stage,
+                     std::vector<RowLaunch> &rows, double *flops) {
+    const ContractionProdType *tij = bp->cptype + cp_ij, *tkl = bp->cptype + cp_kl;
+    int li = tij->l_bra, lj = tij->l_ket, lk = tkl->l_bra, ll = tkl->l_ket;
+    if (li > BQC_LMAX || lj > BQC_LMAX || lk > BQC_LMAX || ll > BQC_LMAX || lj > li || ll > lk)
+        return 1;
+    int pij = pair_id(li, lj), pkl = pair_id(lk, ll);
+    bool swap = pij < pkl;
+    RowKernel kern = swap ? g_rows[pkl][pij] : g_rows[pij][pkl];
+    if (!kern || !bra_order) return -1;
+    const int *pbins = swap ? bins_kl : bins_ij;
+    const int *qbins = swap ? bins_ij : bins_kl;
+    int cp_p = swap ? cp_kl : cp_ij, cp_q = swap ? cp_ij : cp_kl;
+    const int *locs = bp->bas_pairs_locs;
+    int np = pbins[nbins], nq = qbins[nbins];
+    if (pbins[0] != 0 || qbins[0] != 0) return 1;
+    // quartet count for the FLOP model
+    double nquart = 0;
+    for (int b = 0; b < nbins; ++b) {
+        int nkl = bins_kl[b + 1] - bins_kl[b];
+        int nij = bins_ij[nbins - b];
+        if (nkl <= 0 || nij <= 0) continue;
+        if (cp_ij == cp_kl) {
+            for (int kl = bins_kl[b]; kl < bins_kl[b + 1]; ++kl) nquart += nij > kl ? nij - kl : 0;
+        } else {
+            nquart += (double)nkl * nij;
+        }
+    }
+    if (flops) *flops += nquart * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
+    if (np == 0 || nq == 0) return 0;
+    RowLaunch r;
+    memset(&r, 0, sizeof(r));
+    r.kern = kern;
+    V2Task &tk = r.task;
+    tk.np = np;
+    tk.bas_p0 = locs[cp_p];
+    tk.bas_q0 = locs[cp_q];
+    tk.prim_p0 = bp->primitive_pairs_locs[cp_p];
+    tk.prim_q0 = bp->primitive_pairs_locs[cp_q];
+    tk.nprim_p = bp->cptype[cp_p].nprim_12;
+    tk.nprim_q = bp->cptype[cp_q].nprim_12;
+    tk.diag = cp_ij == cp_kl;
+    tk.nbins = nbins;
+    for (int b = 0; b <= nbins; ++b) {
+        tk.pbins[b] = pbins[b];
+        tk.qbins[b] = qbins[b];
+    }
+    r.off_perm_p = stage.size();
+    side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], stage);
+    if (cp_p == cp_q) {
+        r.off_perm_q = r.off_perm_p;
+    } else {
+        r.off_perm_q = stage.size();
+        side_order(qbins, nbins, bra_order + locs[cp_q], locs[cp_q + 1] - locs[cp_q], stage);
+    }
+    if (stage.size() & 1) stage.push_back(0);  // int2 alignment
+    r.off_items = stage.size();
+    int nchunk = (np + 63) / 64;
+    int nitems = 0;
+    for (int c = 0; c < nchunk; ++c) {
+        int first = c * 64;
+        int s = (int)(std::upper_bound(pbins + 1, pbins + nbins + 1, first) - (pbins + 1));
+        int qmax = qbins[nbins - s];
+        for (int q0 = 0; q0 < qmax; q0 += BQC_V2_CK) {
+            stage.push_back(c);
+            stage.push_back(q0);
+            ++nitems;
+        }
+    }
+    tk.nitems = nitems;
+    if (nitems) rows.push_back(r);
+    return 0;
+}
+
 extern "C" {
 
 int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
@@ -139,21 +282,44 @@ int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, in
 
 int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                 int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
-                int precision, const double *log_q, const double *task_dmax,
-                double fp32_cut, hipStream_t stream) {
+                const int *bra_order, int precision, const double *log_q,
+                const double *task_dmax, double fp32_cut, hipStream_t stream) {
     (void)log_q;
     (void)task_dmax;
     (void)fp32_cut;
     if (precision != 0) return 1;
+    if (nbins > BQC_MAX_BINS) return 3;
+    register_kernels();
     JKArgs jk = {vj, vk, dm, nao, n_dm};
     PairData pd = make_pair_data(bpcache);
     double flops = 0;
+    std::vector<RowLaunch> rows;
+    std::vector<int> stage;
     for (int t = 0; t < ntask; ++t) {
         const int *bi = task_bins + (size_t)t * 2 * (nbins + 1);
         const int *bk = bi + nbins + 1;
-        int err = launch_task(bpcache, jk, pd, bi, bk, nbins, task_cp[2 * t],
-                              task_cp[2 * t + 1], stream, &flops);
+        int cp_ij = task_cp[2 * t], cp_kl = task_cp[2 * t + 1];
+        int err = plan_rows(bpcache, bi, bk, nbins, cp_ij, cp_kl, bra_order, stage, rows, &flops);
+        if (err < 0) {
+            err = launch_task(bpcache, jk, pd, bi, bk, nbins, cp_ij, cp_kl, stream, &flops);
+        }
+        if (err > 0) return err;
+    }
+    if (!rows.empty()) {
+        int err = stage_to_device(stage, stream);
         if (err) return err;
+        for (RowLaunch &r : rows) {
+            r.task.perm_p = g_dws + r.off_perm_p;
+            r.task.perm_q = g_dws + r.off_perm_q;
+            r.task.items = (const int2 *)(g_dws + r.off_items);
+            unsigned nblk = (unsigned)((r.task.nitems + 3) / 4);
+            hipLaunchKernelGGL(r.kern, dim3(nblk), dim3(256), 0, stream, jk, pd, r.task);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                fprintf(stderr, "bqcjk: row kernel launch failed: %s\n", hipGetErrorString(e));
+                return 1;
+            }
+        }
     }
     g_last_flops[0] = flops;
     g_last_flops[1] = 0;

@@ -1,5 +1,5 @@
 // Kernel instantiations of the J/K build for bra class (10| and all
 // smaller ket classes.
-#include "jk_kernel.h"
+#include "jk_v2.h"
 
-void bqc_register_bra_10(JKKernelTable &tab) { reg_bra<1, 0>(tab); }
+void bqc_register_bra_10(JKKernelTable &t1, RowKernelTable &t2) { reg_bra<1, 0>(t1, t2); }

@@ -1,5 +1,5 @@
 // Kernel instantiations of the J/K build for bra class (20| and all
 // smaller ket classes.
-#include "jk_kernel.h"
+#include "jk_v2.h"
 
-void bqc_register_bra_20(JKKernelTable &tab) { reg_bra<2, 0>(tab); }
+void bqc_register_bra_20(JKKernelTable &t1, RowKernelTable &t2) { reg_bra<2, 0>(t1, t2); }

@@ -1,5 +1,5 @@
 // Kernel instantiations of the J/K build for bra class (21| and all
 // smaller ket classes.
-#include "jk_kernel.h"
+#include "jk_v2.h"
 
-void bqc_register_bra_21(JKKernelTable &tab) { reg_bra<2, 1>(tab); }
+void bqc_register_bra_21(JKKernelTable &t1, RowKernelTable &t2) { reg_bra<2, 1>(t1, t2); }

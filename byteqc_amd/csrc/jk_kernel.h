@@ -267,18 +267,3 @@ static void reg_kernel(JKKernelTable &tab) {
     tab[LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_quartets<LA, LB, LC, LD>;
 }
 
-// all ket classes (LC LD) with pair-id(LC,LD) <= pair-id(LA,LB)
-template <int LA, int LB>
-static void reg_bra(JKKernelTable &tab) {
-    constexpr int pab = LA * (LA + 1) / 2 + LB;
-    reg_kernel<LA, LB, 0, 0>(tab);
-    if constexpr (pab >= 1) reg_kernel<LA, LB, 1, 0>(tab);
-    if constexpr (pab >= 2) reg_kernel<LA, LB, 1, 1>(tab);
-    if constexpr (pab >= 3) reg_kernel<LA, LB, 2, 0>(tab);
-    if constexpr (pab >= 4) reg_kernel<LA, LB, 2, 1>(tab);
-    if constexpr (pab >= 5) reg_kernel<LA, LB, 2, 2>(tab);
-    if constexpr (pab >= 6) reg_kernel<LA, LB, 3, 0>(tab);
-    if constexpr (pab >= 7) reg_kernel<LA, LB, 3, 1>(tab);
-    if constexpr (pab >= 8) reg_kernel<LA, LB, 3, 2>(tab);
-    if constexpr (pab >= 9) reg_kernel<LA, LB, 3, 3>(tab);
-}

@@ -1,0 +1,301 @@
+// Wavefront-stationary J/K kernel ("pair-row" digestion).
+//
+// One wavefront owns 64 consecutive pairs (a b) of the P list -- the pair
+// type with the larger class -- and walks a range of the Q list (c d) in
+// lock-step: at every step all 64 lanes share the same (c d) pair.  Both
+// lists are in "bin-major, bra-shell-minor" order: the Schwarz bins of the
+// task stay contiguous (so the quartet set is exactly the reference's: a P
+// pair of bin s meets the Q pairs of bins [0, nbins - s)), and inside a bin
+// pairs with the same first shell are adjacent.
+//
+// Digestion of the integral block g(ab|cd) at each step (V symmetrised at the
+// end, so a target may be written as V[x][y] or V[y][x]):
+//   J(a,b) += g D(c,d)    lane register accumulator, flushed once per item
+//   J(c,d) += g D(a,b)    summed over the wavefront, one atomic per component
+//   K(a,c) += g D(b,d)    lane register accumulator over a run of equal c,
+//                         flushed by a segmented sum over runs of equal a
+//   K(a,d) += g D(b,c)    segmented sum over runs of equal a at every step
+//   K(b,c) += g D(a,d)    lane register accumulator over a run of equal c
+//   K(b,d) += g D(a,c)    one atomic per lane and component (smallest block)
+#pragma once
+#include "jk_kernel.h"
+
+#define BQC_V2_CK 64
+
+struct V2Task {
+    const int *perm_p;  // list position -> plan position, P side
+    const int *perm_q;
+    const int2 *items;  // (chunk, q0)
+    int nitems;
+    int np;             // P pairs in bins
+    int bas_p0, bas_q0, prim_p0, prim_q0, nprim_p, nprim_q;
+    int diag;
+    int nbins;
+    int pbins[BQC_MAX_BINS + 1];
+    int qbins[BQC_MAX_BINS + 1];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// sum of v over lanes [lane, run_end) delivered to every lane; the run head
+// (first lane of its run) holds the full run sum
+__device__ __forceinline__ double run_sum(double v, int lane, int run_end) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_down(v, d, 64);
+        if (lane + d < run_end) v += o;
+    }
+    return v;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk) {
+    using C = Cls<LA, LB, LC, LD>;
+    constexpr int NA = C::NFI, NB = C::NFJ, NC = C::NFK, ND = C::NFL;
+    const int lane = threadIdx.x & 63;
+    const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (item >= tk.nitems) return;
+    const int2 it = tk.items[item];
+    const int chunk = __builtin_amdgcn_readfirstlane(it.x);
+    const int q0 = __builtin_amdgcn_readfirstlane(it.y);
+    const int pidx = chunk * 64 + lane;
+    const bool pvalid = pidx < tk.np;
+    // bin of this P pair -> end of its Q prefix
+    int qlim = 0, ppos = 0;
+    if (pvalid) {
+        int s = 0;
+        while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
+        qlim = tk.qbins[tk.nbins - s];
+        ppos = tk.perm_p[pidx];
+    }
+    const int qend_item = min(q0 + BQC_V2_CK, tk.qbins[tk.nbins]);
+    int qlim_max = qlim;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) qlim_max = max(qlim_max, __shfl_xor(qlim_max, d, 64));
+    const int q1 = min(qend_item, __builtin_amdgcn_readfirstlane(qlim_max));
+    if (q1 <= q0) return;
+
+    const int gp = tk.bas_p0 + ppos;
+    const int sa = pvalid ? pd.bra[gp] : -1;
+    const int sb = pvalid ? pd.ket[gp] : 0;
+    const int pp0 = tk.prim_p0 + ppos * tk.nprim_p;
+    const int a0 = pvalid ? pd.ao_loc[sa] : 0;
+    const int b0 = pvalid ? pd.ao_loc[sb] : 0;
+    double ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+    if (pvalid) {
+        ra[0] = pd.bas_x[sa]; ra[1] = pd.bas_y[sa]; ra[2] = pd.bas_z[sa];
+        rb[0] = pd.bas_x[sb]; rb[1] = pd.bas_y[sb]; rb[2] = pd.bas_z[sb];
+    }
+    // runs of equal first shell a among the 64 lanes
+    const int key_next = __shfl_down(sa, 1, 64);
+    unsigned long long ends = __ballot(lane == 63 || key_next != sa);
+    const int run_end = lane + 1 + (int)__builtin_ctzll(ends >> lane);
+    const int key_prev = __shfl_up(sa, 1, 64);
+    const bool head = lane == 0 || key_prev != sa;
+
+    const int nao = jk.nao;
+    const size_t nao2 = (size_t)nao * nao;
+    for (int idm = 0; idm < jk.n_dm; ++idm) {
+        const double *__restrict__ D = jk.dm + idm * nao2;
+        double *VJ = jk.vj ? jk.vj + idm * nao2 : nullptr;
+        double *VK = jk.vk ? jk.vk + idm * nao2 : nullptr;
+        double dab[NA * NB];
+#pragma unroll
+        for (int x = 0; x < NA; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) dab[x * NB + y] = pvalid ? D[(size_t)(a0 + x) * nao + b0 + y] : 0.0;
+        double jab[NA * NB], kac[NA * NC], kbc[NB * NC];
+#pragma unroll
+        for (int n = 0; n < NA * NB; ++n) jab[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NA * NC; ++n) kac[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
+
+        int gq = tk.bas_q0 + tk.perm_q[q0];
+        int sc = __builtin_amdgcn_readfirstlane(pd.bra[gq]);
+        for (int q = q0; q < q1; ++q) {
+            const int qpos = __builtin_amdgcn_readfirstlane(tk.perm_q[q]);
+            gq = tk.bas_q0 + qpos;
+            const int sd = __builtin_amdgcn_readfirstlane(pd.ket[gq]);
+            const int c0 = pd.ao_loc[sc], d0 = pd.ao_loc[sd];
+            bool active = pvalid && q < qlim;
+            double norm = TWO_PI_52;
+            if (tk.diag) {
+                active = active && ppos >= qpos;
+                if (ppos == qpos) norm *= 0.5;
+            }
+            double g[C::NF];
+            if (active) {
+                const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
+                const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
+                eri_block<LA, LB, LC, LD>(g, pd, pp0, tk.nprim_p, tk.prim_q0 + qpos * tk.nprim_q,
+                                          tk.nprim_q, norm, ra, rb, rc, rd);
+            } else {
+#pragma unroll
+                for (int n = 0; n < C::NF; ++n) g[n] = 0.0;
+            }
+#define G(x, y, z, w) g[(((x) * NB + (y)) * NC + (z)) * ND + (w)]
+            if (VJ) {
+                // J(a,b) += g D(c,d)
+#pragma unroll
+                for (int z = 0; z < NC; ++z)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) {
+                        const double dcd = D[(size_t)(c0 + z) * nao + d0 + w];
+#pragma unroll
+                        for (int x = 0; x < NA; ++x)
+#pragma unroll
+                            for (int y = 0; y < NB; ++y) jab[x * NB + y] += G(x, y, z, w) * dcd;
+                    }
+                // J(c,d) += g D(a,b), whole wavefront
+#pragma unroll
+                for (int z = 0; z < NC; ++z)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int x = 0; x < NA; ++x)
+#pragma unroll
+                            for (int y = 0; y < NB; ++y) s += G(x, y, z, w) * dab[x * NB + y];
+                        s = wave_sum(s);
+                        if (lane == 0) atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, s);
+                    }
+            }
+            if (VK) {
+                // K(a,c) += g D(b,d); K(b,c) += g D(a,d); K(a,d) += g D(b,c); K(b,d) += g D(a,c)
+                double dbd[NB * ND], dad[NA * ND], dbc[NB * NC], dac[NA * NC];
+#pragma unroll
+                for (int w = 0; w < ND; ++w) {
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+                        dbd[y * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + b0 + y] : 0.0;
+#pragma unroll
+                    for (int x = 0; x < NA; ++x)
+                        dad[x * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + a0 + x] : 0.0;
+                }
+#pragma unroll
+                for (int z = 0; z < NC; ++z) {
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+                        dbc[y * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + b0 + y] : 0.0;
+#pragma unroll
+                    for (int x = 0; x < NA; ++x)
+                        dac[x * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + a0 + x] : 0.0;
+                }
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+#pragma unroll
+                        for (int z = 0; z < NC; ++z)
+#pragma unroll
+                            for (int w = 0; w < ND; ++w) {
+                                const double v = G(x, y, z, w);
+                                kac[x * NC + z] += v * dbd[y * ND + w];
+                                kbc[y * NC + z] += v * dad[x * ND + w];
+                            }
+                // K(a,d): segmented over runs of equal a
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int y = 0; y < NB; ++y)
+#pragma unroll
+                            for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dbc[y * NC + z];
+                        s = run_sum(s, lane, run_end);
+                        if (head && pvalid) atomic_add(VK + (size_t)(d0 + w) * nao + a0 + x, s);
+                    }
+                // K(b,d): per lane
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int x = 0; x < NA; ++x)
+#pragma unroll
+                            for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dac[x * NC + z];
+                        if (active) atomic_add(VK + (size_t)(d0 + w) * nao + b0 + y, s);
+                    }
+            }
+#undef G
+            // end of a run of equal c: flush K(a,c), K(b,c)
+            int sc_next = -1;
+            if (q + 1 < q1) {
+                sc_next = __builtin_amdgcn_readfirstlane(
+                    pd.bra[tk.bas_q0 + tk.perm_q[q + 1]]);
+            }
+            if (VK && sc_next != sc) {
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) {
+                        const double s = run_sum(kac[x * NC + z], lane, run_end);
+                        if (head && pvalid) atomic_add(VK + (size_t)(c0 + z) * nao + a0 + x, s);
+                        kac[x * NC + z] = 0.0;
+                    }
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) {
+                        if (pvalid && kbc[y * NC + z] != 0.0)
+                            atomic_add(VK + (size_t)(c0 + z) * nao + b0 + y, kbc[y * NC + z]);
+                        kbc[y * NC + z] = 0.0;
+                    }
+            }
+            sc = sc_next;
+        }
+        if (VJ && pvalid) {
+#pragma unroll
+            for (int x = 0; x < NA; ++x)
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+                    if (jab[x * NB + y] != 0.0)
+                        atomic_add(VJ + (size_t)(a0 + x) * nao + b0 + y, jab[x * NB + y]);
+        }
+    }
+}
+
+typedef void (*RowKernel)(JKArgs, PairData, V2Task);
+typedef RowKernel RowKernelTable[BQC_NPAIRID][BQC_NPAIRID];
+
+template <int LA, int LB, int LC, int LD>
+static void reg_row(RowKernelTable &tab) {
+    tab[LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD>;
+}
+
+// Classes digested with the wavefront-stationary kernel (register budget:
+// the integral block and the lane accumulators must stay in VGPRs).
+template <int LA, int LB, int LC, int LD>
+constexpr bool use_rows() {
+    return Cls<LA, LB, LC, LD>::NF <= 27;
+}
+
+template <int LA, int LB, int LC, int LD>
+static void reg_class(JKKernelTable &t1, RowKernelTable &t2) {
+    reg_kernel<LA, LB, LC, LD>(t1);
+    if constexpr (use_rows<LA, LB, LC, LD>()) reg_row<LA, LB, LC, LD>(t2);
+}
+
+// all ket classes (LC LD) with pair-id(LC,LD) <= pair-id(LA,LB)
+template <int LA, int LB>
+static void reg_bra(JKKernelTable &t1, RowKernelTable &t2) {
+    constexpr int pab = LA * (LA + 1) / 2 + LB;
+    reg_class<LA, LB, 0, 0>(t1, t2);
+    if constexpr (pab >= 1) reg_class<LA, LB, 1, 0>(t1, t2);
+    if constexpr (pab >= 2) reg_class<LA, LB, 1, 1>(t1, t2);
+    if constexpr (pab >= 3) reg_class<LA, LB, 2, 0>(t1, t2);
+    if constexpr (pab >= 4) reg_class<LA, LB, 2, 1>(t1, t2);
+    if constexpr (pab >= 5) reg_class<LA, LB, 2, 2>(t1, t2);
+    if constexpr (pab >= 6) reg_class<LA, LB, 3, 0>(t1, t2);
+    if constexpr (pab >= 7) reg_class<LA, LB, 3, 1>(t1, t2);
+    if constexpr (pab >= 8) reg_class<LA, LB, 3, 2>(t1, t2);
+    if constexpr (pab >= 9) reg_class<LA, LB, 3, 3>(t1, t2);
+}

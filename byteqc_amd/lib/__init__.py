@@ -23,8 +23,8 @@ _SIGNATURES = {
     'GINTbuild_jk': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
                              c_void_p, c_int, c_int, c_int, c_void_p]),
     'BQCbuild_jk': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
-                            c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_double,
-                            c_void_p]),
+                            c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                            c_double, c_void_p]),
     'BQCq_cond': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int,
                           c_void_p, c_void_p]),
     'BQCsymmetrize_jk': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),

@@ -280,6 +280,10 @@ class _VHFOpt:
             numpy.vstack([numpy.hstack(bra), numpy.hstack(ket)]).astype(numpy.int32))
         self.bas_pairs_locs = numpy.append(0, numpy.cumsum([len(x) for x in bra])).astype(
             numpy.int32)
+        # per pair type: pair positions ordered by first shell (stable)
+        self.bra_order = numpy.ascontiguousarray(numpy.concatenate(
+            [numpy.argsort(b, kind='stable') for b in bra] + [numpy.zeros(0, int)]).astype(
+                numpy.int32))
         self._init_bpcache(0.5 if not diag_block_with_triu else 1., dev)
         self.build_time = time.perf_counter() - t0
         return self
@@ -411,7 +415,7 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None,
     if len(task_cp):
         err = lib.BQCbuild_jk(vhfopt.bpcache, _ptr(vj), _ptr(vk), _ptr(dmc), nao, n_dm,
                               len(task_cp), _ptr(task_cp), _ptr(task_bins), nbins,
-                              PRECISION[precision], _ptr(vhfopt._dev['log_q']),
+                              _ptr(vhfopt.bra_order), PRECISION[precision], _ptr(vhfopt._dev['log_q']),
                               _ptr(task_dmax), fp32_cut, s)
         if err:
             bad = []

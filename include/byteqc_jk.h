@@ -98,7 +98,11 @@ void BQCdel_basis_prod_clone(BasisProdCache **pbp);
 
 /* All tasks of one J/K build in one call.  task_cp (host, 2 x ntask):
  * (cp_ij, cp_kl) per task; task_bins (host, ntask x 2 x (nbins+1)): ij bins
- * then kl bins per task, as for GINTbuild_jk.  precision: 0 = FP64
+ * then kl bins per task, as for GINTbuild_jk.  bra_order (host, one entry
+ * per pair, or NULL): for every pair type, the positions of its pairs in
+ * the pair list ordered by first shell (stable); it lets the kernels walk
+ * each Schwarz bin grouped by first shell.  Builds that share a library
+ * must be ordered on one stream (they share a device work list).  precision: 0 = FP64
  * throughout; 1 = mixed (quartets whose bound q_ij q_kl dmax falls below
  * fp32_cut are evaluated in FP32 after FP64 Rys roots; accumulation FP64).
  * q_ij for the mixed rule comes from log_q (device, one per pair, same order
@@ -107,8 +111,8 @@ void BQCdel_basis_prod_clone(BasisProdCache **pbp);
  * 3 too many bins. */
 int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                 int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
-                int precision, const double *log_q, const double *task_dmax,
-                double fp32_cut, hipStream_t stream);
+                const int *bra_order, int precision, const double *log_q,
+                const double *task_dmax, double fp32_cut, hipStream_t stream);
 
 /* Schwarz bounds on the GPU: q[i*nbas + j] = sqrt(max |(ij|ij)|) over the
  * Cartesian components of shells i, j (1e-100 when zero), for the shell
