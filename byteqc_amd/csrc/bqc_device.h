@@ -92,6 +92,11 @@ struct Cls {
     // index of the 2-D integral I(i, j, k, l) of one direction
     static constexpr int S_L = 1, S_K = LL + 1, S_J = (LK + 1) * S_K, S_I = (LJ + 1) * S_J;
     static constexpr int GSZ = (LI + 1) * S_I;
+    // classes whose integral block is too large for registers use rolled
+    // outer loops (the block then lives in scratch memory)
+    static constexpr bool BIG = NF > 160;
+    static constexpr int UO = BIG ? 1 : 16;
+    static constexpr int UN = BIG ? 8 : 2048;
     __host__ __device__ static constexpr int g(int i, int j, int k, int l) {
         return i * S_I + j * S_J + k * S_K + l;
     }

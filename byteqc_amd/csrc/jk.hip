@@ -26,6 +26,10 @@ BQC_DECL_BRA(1, 1)
 BQC_DECL_BRA(2, 0)
 BQC_DECL_BRA(2, 1)
 BQC_DECL_BRA(2, 2)
+BQC_DECL_BRA(3, 0)
+BQC_DECL_BRA(3, 1)
+BQC_DECL_BRA(3, 2)
+BQC_DECL_BRA(3, 3)
 
 static void register_kernels() {
     if (g_registered) return;
@@ -35,10 +39,38 @@ static void register_kernels() {
     bqc_register_bra_20(g_kernels, g_rows);
     bqc_register_bra_21(g_kernels, g_rows);
     bqc_register_bra_22(g_kernels, g_rows);
+    bqc_register_bra_30(g_kernels, g_rows);
+    bqc_register_bra_31(g_kernels, g_rows);
+    bqc_register_bra_32(g_kernels, g_rows);
+    bqc_register_bra_33(g_kernels, g_rows);
     g_registered = true;
 }
 
 static double g_last_flops[2] = {0, 0};
+
+// Optional per-launch timing: HIP events around every kernel of a build,
+// on the build's stream.  Records (class code, algorithmic FLOPs, events).
+struct LaunchRec {
+    int cls;       // li*1000 + lj*100 + lk*10 + ll of the evaluated class
+    int kind;      // 0 quartet kernel, 1 row kernel
+    double flops;  // algorithmic FLOPs of the launch
+    double nquart;
+    hipEvent_t e0, e1;
+};
+static bool g_prof = false;
+static std::vector<LaunchRec> g_recs;
+static std::vector<hipEvent_t> g_evpool;
+
+static hipEvent_t prof_event() {
+    if (g_evpool.empty()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t e = g_evpool.back();
+    g_evpool.pop_back();
+    return e;
+}
 
 // Algorithmic FLOP model of one contracted quartet (K primitive quartets).
 static double quartet_flops(int li, int lj, int lk, int ll, int K) {
@@ -97,13 +129,28 @@ static int launch_task(BasisProdCache *bp, const JKArgs &jk, const PairData &pd,
     ta.nbins = nb;
     if (nb == 0) return 0;
     if (bins_ij[0] != 0) return 1;
-    if (flops)
-        *flops += nq * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
+    const double lflops = nq * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
+    if (flops) *flops += lflops;
     JKKernel kern = ta.swap ? g_kernels[pkl][pij] : g_kernels[pij][pkl];
     if (!kern) return 1;
     long long ntot = ta.pre[nb];
     long long nblk = (ntot + BQC_THREADS - 1) / BQC_THREADS;
+    LaunchRec rec;
+    if (g_prof) {
+        int a = ta.swap ? 1 : 0;
+        rec.cls = a ? lk * 1000 + ll * 100 + li * 10 + lj : li * 1000 + lj * 100 + lk * 10 + ll;
+        rec.kind = 0;
+        rec.flops = lflops;
+        rec.nquart = nq;
+        rec.e0 = prof_event();
+        rec.e1 = prof_event();
+        hipEventRecord(rec.e0, stream);
+    }
     hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, ta);
+    if (g_prof) {
+        hipEventRecord(rec.e1, stream);
+        g_recs.push_back(rec);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", li, lj, lk, ll,
@@ -136,6 +183,8 @@ struct RowLaunch {
     RowKernel kern;
     V2Task task;
     size_t off_perm_p, off_perm_q, off_items;
+    int cls;
+    double flops, nquart;
 };
 
 static int *g_dws = nullptr;         // device workspace (ints)
@@ -223,11 +272,15 @@ static int plan_rows(BasisProdCache *bp, const int *bins_ij, const int *bins_kl,
             nquart += (double)nkl * nij;
         }
     }
-    if (flops) *flops += nquart * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
+    const double lflops = nquart * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
+    if (flops) *flops += lflops;
     if (np == 0 || nq == 0) return 0;
     RowLaunch r;
     memset(&r, 0, sizeof(r));
     r.kern = kern;
+    r.cls = swap ? lk * 1000 + ll * 100 + li * 10 + lj : li * 1000 + lj * 100 + lk * 10 + ll;
+    r.flops = lflops;
+    r.nquart = nquart;
     V2Task &tk = r.task;
     tk.np = np;
     tk.bas_p0 = locs[cp_p];
@@ -313,7 +366,21 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
             r.task.perm_q = g_dws + r.off_perm_q;
             r.task.items = (const int2 *)(g_dws + r.off_items);
             unsigned nblk = (unsigned)((r.task.nitems + 3) / 4);
+            LaunchRec rec;
+            if (g_prof) {
+                rec.cls = r.cls;
+                rec.kind = 1;
+                rec.flops = r.flops;
+                rec.nquart = r.nquart;
+                rec.e0 = prof_event();
+                rec.e1 = prof_event();
+                hipEventRecord(rec.e0, stream);
+            }
             hipLaunchKernelGGL(r.kern, dim3(nblk), dim3(256), 0, stream, jk, pd, r.task);
+            if (g_prof) {
+                hipEventRecord(rec.e1, stream);
+                g_recs.push_back(rec);
+            }
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) {
                 fprintf(stderr, "bqcjk: row kernel launch failed: %s\n", hipGetErrorString(e));
@@ -324,6 +391,35 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
     g_last_flops[0] = flops;
     g_last_flops[1] = 0;
     return 0;
+}
+
+void BQCprofile_enable(int on) {
+    g_prof = on != 0;
+}
+
+int BQCprofile_count(void) { return (int)g_recs.size(); }
+
+// Drains the launch records: cls, kind, flops, quartets and duration (ms)
+// per launch; waits for the recorded events.  Returns the number written.
+int BQCprofile_read(int *cls, int *kind, double *flops, double *nquart, double *ms, int cap) {
+    int n = 0;
+    for (LaunchRec &r : g_recs) {
+        if (n < cap) {
+            hipEventSynchronize(r.e1);
+            float t = 0;
+            hipEventElapsedTime(&t, r.e0, r.e1);
+            cls[n] = r.cls;
+            kind[n] = r.kind;
+            flops[n] = r.flops;
+            nquart[n] = r.nquart;
+            ms[n] = t;
+            ++n;
+        }
+        g_evpool.push_back(r.e0);
+        g_evpool.push_back(r.e1);
+    }
+    g_recs.clear();
+    return n;
 }
 
 void BQClast_build_flops(double *flops) {

@@ -1,0 +1,5 @@
+// Kernel instantiations of the J/K build for bra class (33| and all
+// smaller ket classes.
+#include "jk_v2.h"
+
+void bqc_register_bra_33(JKKernelTable &t1, RowKernelTable &t2) { reg_bra<3, 3>(t1, t2); }

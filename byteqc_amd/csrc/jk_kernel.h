@@ -71,7 +71,7 @@ __device__ __forceinline__ void eri_block(double *__restrict__ gout, const PairD
                                           const double *rc, const double *rd) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NR = C::NR;
-#pragma unroll
+#pragma unroll C::UN
     for (int n = 0; n < C::NF; ++n) gout[n] = 0.0;
     const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
     const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
@@ -110,18 +110,24 @@ __device__ __forceinline__ void eri_block(double *__restrict__ gout, const PairD
                 rys2d<LA, LB, LC, LD, double>(Iz, w[r], (P[2] - ra[2]) - tmp2 * pq[2],
                                               (Q[2] - rc[2]) + tmp3 * pq[2], b10, b01, b00,
                                               ab[2], cd[2]);
+#pragma unroll C::UO
+                for (int x = 0; x < C::NFI; ++x) {
+                    const int ax = cart_x(LA, x), ay = cart_y(LA, x), az = cart_z(LA, x);
+#pragma unroll C::UO
+                    for (int y = 0; y < C::NFJ; ++y) {
+                        const int bx = cart_x(LB, y), by = cart_y(LB, y), bz = cart_z(LB, y);
 #pragma unroll
-                for (int n = 0; n < C::NF; ++n) {
-                    const int ci = n / (C::NFJ * C::NFK * C::NFL);
-                    const int cj = (n / (C::NFK * C::NFL)) % C::NFJ;
-                    const int ck = (n / C::NFL) % C::NFK;
-                    const int cl = n % C::NFL;
-                    gout[n] += Ix[C::g(cart_x(LA, ci), cart_x(LB, cj), cart_x(LC, ck),
-                                       cart_x(LD, cl))] *
-                               Iy[C::g(cart_y(LA, ci), cart_y(LB, cj), cart_y(LC, ck),
-                                       cart_y(LD, cl))] *
-                               Iz[C::g(cart_z(LA, ci), cart_z(LB, cj), cart_z(LC, ck),
-                                       cart_z(LD, cl))];
+                        for (int z = 0; z < C::NFK; ++z) {
+                            const int cx = cart_x(LC, z), cy = cart_y(LC, z), cz = cart_z(LC, z);
+#pragma unroll
+                            for (int w = 0; w < C::NFL; ++w) {
+                                gout[((x * C::NFJ + y) * C::NFK + z) * C::NFL + w] +=
+                                    Ix[C::g(ax, bx, cx, cart_x(LD, w))] *
+                                    Iy[C::g(ay, by, cy, cart_y(LD, w))] *
+                                    Iz[C::g(az, bz, cz, cart_z(LD, w))];
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -139,7 +145,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
         const double *__restrict__ D = jk.dm + idm * nao2;
         if (jk.vj) {
             double *V = jk.vj + idm * nao2;
-#pragma unroll
+#pragma unroll C::UO
             for (int ia = 0; ia < C::NFI; ++ia)
 #pragma unroll
                 for (int ib = 0; ib < C::NFJ; ++ib) {
@@ -152,12 +158,12 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                                  D[(c0 + ic) * nao + d0 + id];
                     atomic_add(V + (size_t)(a0 + ia) * nao + b0 + ib, s);
                 }
-#pragma unroll
+#pragma unroll C::UO
             for (int ic = 0; ic < C::NFK; ++ic)
 #pragma unroll
                 for (int id = 0; id < C::NFL; ++id) {
                     double s = 0.0;
-#pragma unroll
+#pragma unroll C::UO
                     for (int ia = 0; ia < C::NFI; ++ia)
 #pragma unroll
                         for (int ib = 0; ib < C::NFJ; ++ib)
@@ -169,7 +175,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
         if (jk.vk) {
             double *V = jk.vk + idm * nao2;
             // V[a,c] += g D[b,d]; V[a,d] += g D[b,c]
-#pragma unroll
+#pragma unroll C::UO
             for (int ia = 0; ia < C::NFI; ++ia) {
 #pragma unroll
                 for (int ic = 0; ic < C::NFK; ++ic) {
@@ -195,12 +201,12 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                 }
             }
             // V[b,c] += g D[a,d]; V[b,d] += g D[a,c]
-#pragma unroll
+#pragma unroll C::UO
             for (int ib = 0; ib < C::NFJ; ++ib) {
 #pragma unroll
                 for (int ic = 0; ic < C::NFK; ++ic) {
                     double s = 0.0;
-#pragma unroll
+#pragma unroll C::UO
                     for (int ia = 0; ia < C::NFI; ++ia)
 #pragma unroll
                         for (int id = 0; id < C::NFL; ++id)
@@ -211,7 +217,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
 #pragma unroll
                 for (int id = 0; id < C::NFL; ++id) {
                     double s = 0.0;
-#pragma unroll
+#pragma unroll C::UO
                     for (int ia = 0; ia < C::NFI; ++ia)
 #pragma unroll
                         for (int ic = 0; ic < C::NFK; ++ic)

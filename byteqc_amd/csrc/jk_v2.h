@@ -275,7 +275,7 @@ static void reg_row(RowKernelTable &tab) {
 // the integral block and the lane accumulators must stay in VGPRs).
 template <int LA, int LB, int LC, int LD>
 constexpr bool use_rows() {
-    return Cls<LA, LB, LC, LD>::NF <= 27;
+    return Cls<LA, LB, LC, LD>::NF <= 81;
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -29,6 +29,9 @@ _SIGNATURES = {
                           c_void_p, c_void_p]),
     'BQCsymmetrize_jk': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
     'BQClast_build_flops': (None, [c_void_p]),
+    'BQCprofile_enable': (None, [c_int]),
+    'BQCprofile_count': (c_int, []),
+    'BQCprofile_read': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
 }
 
 

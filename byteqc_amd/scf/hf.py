@@ -175,6 +175,19 @@ def basis_seg_contraction(mol, allow_replica=False):
     return _PrimMol(mol, numpy.vstack(bas_out), env), contr
 
 
+def _morton(env, atm, bas, bits=10):
+    """Morton (Z-order) key of every shell's atom position."""
+    xyz = numpy.stack([env[atm[a, 1]:atm[a, 1] + 3] for a in bas[:, ATOM_OF]])
+    lo = xyz.min(0)
+    span = max(float((xyz.max(0) - lo).max()), 1e-12)
+    q = numpy.minimum(((xyz - lo) / span * (2 ** bits - 1)).astype(numpy.int64), 2 ** bits - 1)
+    key = numpy.zeros(len(bas), dtype=numpy.int64)
+    for b in range(bits):
+        for d in range(3):
+            key |= ((q[:, d] >> b) & 1) << (3 * b + d)
+    return key
+
+
 def _make_s_index_offsets(log_q, nbins=10, cutoff=1e-12):
     """Split a pair list sorted by descending log q into ``nbins`` buckets
     down to ``cutoff``; return the (nbins+1) bucket offsets."""
@@ -232,7 +245,13 @@ class _VHFOpt:
         if group_size is not None:
             uniq_l_ctr, l_ctr_counts = _split_l_ctr_groups(uniq_l_ctr, l_ctr_counts,
                                                            group_size)
-        sorted_idx = numpy.argsort(inv_idx, kind='stable').astype(numpy.int32)
+        # group by (l, nprim) as the reference does (hf.py:296-320); inside a
+        # group order shells along a Morton curve of their atom positions so
+        # that partners of a shell have nearby AO indices (the quartet set
+        # does not depend on this order)
+        key = _morton(mol._env, mol._atm, mol._bas)
+        sorted_idx = numpy.lexsort((numpy.arange(len(inv_idx)), key, inv_idx)).astype(
+            numpy.int32)
         ao_loc = mol.ao_loc_nr(cart=True)
         ao_idx = numpy.hstack([numpy.arange(ao_loc[i], ao_loc[i + 1]) for i in sorted_idx])
         self.coeff = self.coeff[ao_idx]
@@ -453,3 +472,22 @@ def last_build_flops():
     out = numpy.zeros(2)
     load_library().BQClast_build_flops(_ptr(out))
     return out
+
+
+def profile_enable(on=True):
+    """Bracket every J/K kernel launch with HIP events (see profile_read)."""
+    load_library().BQCprofile_enable(1 if on else 0)
+
+
+def profile_read():
+    """Drain per-launch records: class code, kernel kind, FLOPs, quartets, ms."""
+    lib = load_library()
+    n = lib.BQCprofile_count()
+    cls = numpy.zeros(n, numpy.int32)
+    kind = numpy.zeros(n, numpy.int32)
+    fl = numpy.zeros(n)
+    nq = numpy.zeros(n)
+    ms = numpy.zeros(n)
+    m = lib.BQCprofile_read(_ptr(cls), _ptr(kind), _ptr(fl), _ptr(nq), _ptr(ms), n)
+    return [dict(cls=int(cls[i]), kind=int(kind[i]), flops=float(fl[i]),
+                 nquart=float(nq[i]), ms=float(ms[i])) for i in range(m)]

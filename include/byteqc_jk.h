@@ -125,6 +125,16 @@ int BQCq_cond(double *q, const int *atm, const int *bas, const double *env, int 
  * (either may be NULL). */
 int BQCsymmetrize_jk(double *vj, double *vk, int nao, int n_dm, hipStream_t stream);
 
+/* Per-launch timing of BQCbuild_jk / GINTbuild_jk: when enabled, every
+ * kernel launch is bracketed by HIP events on the build's stream.
+ * BQCprofile_read drains the records (waiting for their events): per launch
+ * the evaluated class li*1000+lj*100+lk*10+ll, the kernel kind (0 quartet
+ * kernel, 1 row kernel), the algorithmic FLOPs and quartet count of the
+ * launch and its duration in ms; returns the number written (<= cap). */
+void BQCprofile_enable(int on);
+int BQCprofile_count(void);
+int BQCprofile_read(int *cls, int *kind, double *flops, double *nquart, double *ms, int cap);
+
 /* Number of FP64 (and FP32) floating-point operations of the last
  * BQCbuild_jk call's quartet list by the algorithmic model of the path
  * (host counters, written into flops[0..1]). */

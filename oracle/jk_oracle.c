@@ -560,6 +560,8 @@ void oracle_q_cond(double *qc, const int *atm, int natm, const int *bas, int nba
  *   K: V[i,k] += g D[j,l],  V[i,l] += g D[j,k],
  *      V[j,k] += g D[i,l],  V[j,l] += g D[i,k]
  * and the caller forms J = 2 (V + V^T), K = V + V^T (hf.py:171-181).
+ * kl_stride > 1 keeps only the kl pairs with kl % kl_stride == 0 (a uniform
+ * sample of every task, used to time the CPU path on a bounded workload).
  */
 int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
               const int *bas_pairs_locs, const int *prim_pairs_locs,
@@ -567,7 +569,7 @@ int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
               int npairs_total, const double *aexyz, int nprim_pairs_total,
               const int *atm, const int *bas, const double *env, const int *ao_loc,
               int nao, int n_dm, const double *dm, double *vj, double *vk,
-              long long *nquartets_out) {
+              long long *nquartets_out, int kl_stride) {
     if (!orc_ready) oracle_rys_init();
     (void)npairs_total;
     /* flatten (task, kl_bin, kl) work items */
@@ -576,6 +578,7 @@ int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
         const int *bk = task_bins + (size_t)t * 2 * (nbins + 1) + (nbins + 1);
         nitems += bk[nbins] - bk[0];
     }
+    if (kl_stride < 1) kl_stride = 1;
     int *it_task = (int *)malloc(sizeof(int) * (nitems + 1));
     int *it_kl = (int *)malloc(sizeof(int) * (nitems + 1));
     int *it_bin = (int *)malloc(sizeof(int) * (nitems + 1));
@@ -583,10 +586,12 @@ int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
     for (int t = 0; t < ntask; t++) {
         const int *bk = task_bins + (size_t)t * 2 * (nbins + 1) + (nbins + 1);
         for (int b = 0; b < nbins; b++)
-            for (int kl = bk[b]; kl < bk[b + 1]; kl++, c++) {
+            for (int kl = bk[b]; kl < bk[b + 1]; kl++) {
+                if (kl % kl_stride) continue;
                 it_task[c] = t;
                 it_kl[c] = kl;
                 it_bin[c] = b;
+                c++;
             }
     }
     size_t nao2 = (size_t)nao * nao;
@@ -602,7 +607,7 @@ int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
         double *vkp = vjp + (size_t)n_dm * nao2;
         double *blk = (double *)malloc(sizeof(double) * 50625);
 #pragma omp for schedule(dynamic, 16)
-        for (long long item = 0; item < nitems; item++) {
+        for (long long item = 0; item < c; item++) {
             int t = it_task[item], kl = it_kl[item], kl_bin = it_bin[item];
             int cp_ij = task_cp[2 * t], cp_kl = task_cp[2 * t + 1];
             const int *bi = task_bins + (size_t)t * 2 * (nbins + 1);

@@ -270,7 +270,7 @@ def _condense_absmax(d, loc):
 
 
 def get_jk(mol, dm, plan=None, with_j=True, with_k=True, prescreen=True, nbins=None,
-           return_count=False):
+           return_count=False, kl_stride=1):
     """Oracle J and K for hermitian densities dm of shape (nao0,nao0) or (n,nao0,nao0)."""
     if plan is None:
         plan = build_plan(mol)
@@ -279,7 +279,7 @@ def get_jk(mol, dm, plan=None, with_j=True, with_k=True, prescreen=True, nbins=N
     dm0 = numpy.asarray(dm)
     dms = dm0.reshape(-1, nao0, nao0)
     n_dm = len(dms)
-    dmc = numpy.ascontiguousarray(numpy.einsum('pi,nij,qj->npq', coeff, dms, coeff))
+    dmc = numpy.ascontiguousarray(coeff @ dms @ coeff.T)
     if nbins is None:
         pairs_max = int(numpy.diff(plan.bas_pairs_locs).max())
         nbins = max(10, pairs_max // 100000)
@@ -316,18 +316,18 @@ def get_jk(mol, dm, plan=None, with_j=True, with_k=True, prescreen=True, nbins=N
         _ptr(plan.aexyz), ctypes.c_int(plan.nprim_pairs), _ptr(plan.atm), _ptr(plan.bas),
         _ptr(plan.env), _ptr(plan.ao_loc), ctypes.c_int(nao), ctypes.c_int(n_dm),
         _ptr(dmc), _ptr(vj) if with_j else None, _ptr(vk) if with_k else None,
-        ctypes.byref(nq))
+        ctypes.byref(nq), ctypes.c_int(kl_stride))
     if err:
         raise RuntimeError('oracle_jk failed')
     out = []
     if with_j:
         vj = 2 * (vj + vj.transpose(0, 2, 1))
-        out.append(numpy.einsum('pi,npq,qj->nij', coeff, vj, coeff))
+        out.append(coeff.T @ vj @ coeff)
     else:
         out.append(None)
     if with_k:
         vk = vk + vk.transpose(0, 2, 1)
-        out.append(numpy.einsum('pi,npq,qj->nij', coeff, vk, coeff))
+        out.append(coeff.T @ vk @ coeff)
     else:
         out.append(None)
     if dm0.ndim == 2:

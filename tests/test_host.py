@@ -1,0 +1,152 @@
+"""CPU tests of the host side: basis/Mole conventions, the plan helpers that
+mirror the reference's Python (checked against the oracle's independent
+restatement), the Rys table data, and the C-ABI library's exports."""
+import ctypes
+import math
+import os
+import re
+
+import numpy
+import pytest
+
+from byteqc_amd.gto import M, cart2sph, gto_norm, load_basis
+from byteqc_amd import rys_table
+from byteqc_amd.scf import hf
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_cart2sph_matches_libcint_constants():
+    """Real solid harmonics in libcint's convention (no Condon-Shortley
+    phase, m = -l..l), checked against the closed-form normalisation
+    constants sqrt(15/4pi), sqrt(5/16pi), sqrt(15/16pi), sqrt(35/32pi),
+    sqrt(105/4pi), sqrt(21/32pi), sqrt(7/16pi), sqrt(105/16pi)."""
+    pi = math.pi
+    d = cart2sph(2)  # rows xx xy xz yy yz zz
+    assert d[1, 0] == pytest.approx(math.sqrt(15 / (4 * pi)))          # xy
+    assert d[4, 1] == pytest.approx(math.sqrt(15 / (4 * pi)))          # yz
+    assert d[5, 2] == pytest.approx(2 * math.sqrt(5 / (16 * pi)))      # zz
+    assert d[0, 2] == pytest.approx(-math.sqrt(5 / (16 * pi)))
+    assert d[2, 3] == pytest.approx(math.sqrt(15 / (4 * pi)))          # xz
+    assert d[0, 4] == pytest.approx(math.sqrt(15 / (16 * pi)))         # xx - yy
+    assert d[3, 4] == pytest.approx(-math.sqrt(15 / (16 * pi)))
+    f = cart2sph(3)  # rows xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+    c = math.sqrt(35 / (32 * pi))
+    assert f[1, 0] == pytest.approx(3 * c) and f[6, 0] == pytest.approx(-c)
+    assert f[4, 1] == pytest.approx(math.sqrt(105 / (4 * pi)))
+    c = math.sqrt(21 / (32 * pi))
+    assert f[8, 2] == pytest.approx(4 * c) and f[1, 2] == pytest.approx(-c)
+    c = math.sqrt(7 / (16 * pi))
+    assert f[9, 3] == pytest.approx(2 * c) and f[2, 3] == pytest.approx(-3 * c)
+    c = math.sqrt(105 / (16 * pi))
+    assert f[2, 5] == pytest.approx(c) and f[7, 5] == pytest.approx(-c)
+    c = math.sqrt(35 / (32 * pi))
+    assert f[0, 6] == pytest.approx(c) and f[3, 6] == pytest.approx(-3 * c)
+
+
+@pytest.mark.parametrize('l', [2, 3])
+def test_cart2sph_columns_are_orthonormal_harmonics(l):
+    """On the unit sphere the columns are orthonormal (quadrature over a
+    product Gauss-Legendre x uniform-phi grid)."""
+    from byteqc_amd.gto import cart_components
+    t, wt = numpy.polynomial.legendre.leggauss(20)
+    phi = numpy.linspace(0, 2 * math.pi, 40, endpoint=False)
+    ct, ph = numpy.meshgrid(t, phi, indexing='ij')
+    st = numpy.sqrt(1 - ct ** 2)
+    x, y, z = st * numpy.cos(ph), st * numpy.sin(ph), ct
+    w = numpy.outer(wt, numpy.full(phi.size, 2 * math.pi / phi.size))
+    mono = numpy.stack([x ** a * y ** b * z ** c for a, b, c in cart_components(l)], -1)
+    ylm = mono @ cart2sph(l)
+    gram = numpy.einsum('ij,ijm,ijn->mn', w, ylm, ylm)
+    assert numpy.allclose(gram, numpy.eye(2 * l + 1), atol=1e-12)
+
+
+def test_basis_sizes_match_workloads():
+    water = 'O 0 0 0; H 0 0.757 0.587; H 0 -0.757 0.587'
+    assert M(atom=water, basis='cc-pvdz').nao == 24
+    assert M(atom=water, basis='cc-pvtz').nao == 58
+    assert M(atom=water, basis='def2-svp').nao == 24
+    # general contractions kept as one shell: O cc-pVDZ s is 9 primitives x 3
+    o = load_basis('cc-pvdz', 'O')
+    assert o[0][0] == 0 and len(o[0]) - 1 == 9 and len(o[0][1]) == 4
+
+
+def test_contracted_functions_are_normalised():
+    mol = M(atom='O 0 0 0', basis='cc-pvtz')
+    for ib in range(mol.nbas):
+        l = mol.bas_angular(ib)
+        es = mol.bas_exp(ib)
+        cs = mol.bas_ctr_coeff(ib)
+        ee = es[:, None] + es[None, :]
+        s = numpy.vectorize(lambda a: math.gamma(l + 1.5) / (2 * a ** (l + 1.5)))(ee)
+        assert numpy.allclose(numpy.einsum('pi,pq,qi->i', cs, s, cs), 1.)
+
+
+@pytest.mark.parametrize('basis', ['cc-pvdz', 'cc-pvtz', 'def2-svp'])
+def test_seg_contraction_matches_oracle(basis):
+    mol = M(atom='O 0 0 0; H 0 0.757 0.587; H 0 -0.757 0.587; O 3 0 0', basis=basis)
+    pmol, c = hf.basis_seg_contraction(mol)
+    bas, env, c2 = orc.seg_contract(mol)
+    assert numpy.array_equal(pmol._bas, bas)
+    assert numpy.allclose(pmol._env, env, rtol=1e-15)
+    assert numpy.allclose(c, c2, rtol=1e-14, atol=1e-15)
+
+
+def test_s_index_offsets_match_oracle():
+    rng = numpy.random.default_rng(1)
+    for _ in range(20):
+        lq = -numpy.sort(rng.exponential(8, rng.integers(1, 500)))
+        lq[lq > 0] = 0
+        for nbins in (1, 10, 17):
+            cut = 10 ** rng.uniform(-16, -6)
+            assert numpy.array_equal(hf._make_s_index_offsets(lq, nbins, cut),
+                                     orc.make_s_index_offsets(lq, nbins, cut))
+
+
+def test_split_l_ctr_groups():
+    uniq = numpy.array([[0, 1], [1, 1], [2, 1]])
+    counts = numpy.array([10, 7, 3])
+    u, c = hf._split_l_ctr_groups(uniq, counts, group_size=9)
+    assert c.sum() == counts.sum()
+    for (l, _), n in zip(u, c):
+        assert n * (l + 1) * (l + 2) // 2 <= max(9, 2 * (l + 1) * (l + 2) // 2)
+
+
+def test_rys_table_matches_mpmath():
+    gold = numpy.load(os.path.join(ROOT, 'tests', 'golden', 'rys_golden.npz'))
+    for n in range(1, 8):
+        for ix, x in enumerate(gold['x']):
+            u, w = rys_table.rys(n, x)
+            assert numpy.allclose(u, gold[f'u_{n}'][ix], rtol=2e-14, atol=0), (n, x)
+            assert numpy.allclose(w, gold[f'w_{n}'][ix], rtol=2e-14, atol=0), (n, x)
+
+
+def test_library_exports_header_symbols():
+    from byteqc_amd.lib import library_path
+    path = library_path()
+    if not os.path.exists(path):
+        pytest.skip('libbqcjk.so not built')
+    lib = ctypes.CDLL(path)
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, 'include')):
+        text = open(os.path.join(ROOT, 'include', fn)).read()
+        text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+        names.update(re.findall(r'^\s*(?:int|void|double)\s+\**(\w+)\s*\(', text, re.M))
+    assert len(names) >= 10
+    for name in sorted(names):
+        assert hasattr(lib, name), name
+
+
+def test_workload_generator_is_seeded_and_respects_min_distance():
+    from byteqc_amd.systems import water_cluster, random_density
+    a = water_cluster(20, 12.0, 3)
+    b = water_cluster(20, 12.0, 3)
+    assert a == b
+    o = numpy.array([xyz for s, xyz in a if s == 'O'])
+    d = numpy.linalg.norm(o[:, None] - o[None], axis=-1) + numpy.eye(20) * 99
+    assert d.min() >= 2.7
+    h = numpy.array([xyz for s, xyz in a if s == 'H']).reshape(20, 2, 3)
+    assert numpy.allclose(numpy.linalg.norm(h - o[:, None], axis=-1), 0.9572)
+    dm = random_density(30, 5, 1)
+    assert numpy.allclose(dm @ dm, 2 * dm)

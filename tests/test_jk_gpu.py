@@ -26,7 +26,7 @@ def _report(tag, j, k, jref, kref):
     return dj, dk
 
 
-@pytest.mark.parametrize('basis', ['cc-pvdz', 'def2-svp'])
+@pytest.mark.parametrize('basis', ['cc-pvdz', 'def2-svp', 'cc-pvtz'])
 def test_one_water_matches_oracle(basis):
     _torch_gpu()
     from byteqc_amd.gto import M
@@ -99,3 +99,17 @@ def test_hermi_and_omega_errors():
         pass
     with pytest.raises(NotImplementedError):
         hf._get_jk(MF(), mol, dm, omega=0.3)
+
+
+def test_water_dimer_cc_pvtz_matches_oracle():
+    """f shells on both centres: every class up to (ff|ff)."""
+    _torch_gpu()
+    from byteqc_amd.scf import hf
+    from byteqc_amd.systems import water_mole, random_density
+    from oracle import oracle as orc
+    mol = water_mole(2, 4.0, 11, 'cc-pvtz')
+    dm = random_density(mol.nao, 10, 3)
+    j, k = hf.get_jk(mol, dm)
+    jref, kref = orc.get_jk(mol, dm)
+    dj, dk = _report('2 H2O cc-pVTZ', j, k, jref, kref)
+    assert dj < TOL and dk < TOL
