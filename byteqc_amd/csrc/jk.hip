@@ -5,6 +5,7 @@
 // the larger one is evaluated with bra and ket exchanged.  Kernel
 // instantiations live in jk_bra_XY.hip, one translation unit per bra class.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
@@ -52,7 +53,7 @@ static double g_last_flops[2] = {0, 0};
 // on the build's stream.  Records (class code, algorithmic FLOPs, events).
 struct LaunchRec {
     int cls;       // li*1000 + lj*100 + lk*10 + ll of the evaluated class
-    int kind;      // 0 quartet kernel, 1 row kernel
+    int kind;      // 0 quartet kernel, 1 row kernel; +2 for FP32
     double flops;  // algorithmic FLOPs of the launch
     double nquart;
     hipEvent_t e0, e1;
@@ -84,78 +85,170 @@ static double quartet_flops(int li, int lj, int lk, int ll, int K) {
     return K * (20.0 + nr * (60.0 + 20.0 + 3.0 * f2d + 3.0 * nf)) + 12.0 * nf;
 }
 
-static int launch_task(BasisProdCache *bp, const JKArgs &jk, const PairData &pd,
-                       const int *bins_ij, const int *bins_kl, int nbins, int cp_ij,
-                       int cp_kl, hipStream_t stream, double *flops) {
+// Quartets of the rectangle ij in [i0, i1) x kl in [k0, k1); in a task with
+// cp_ij == cp_kl only ij >= kl counts.
+static double count_rect(int i0, int i1, int k0, int k1, bool diag) {
+    if (i1 <= i0 || k1 <= k0) return 0;
+    if (!diag) return (double)(i1 - i0) * (k1 - k0);
+    double n = 0;
+    for (int kl = k0; kl < k1; ++kl) {
+        int lo = i0 > kl ? i0 : kl;
+        if (i1 > lo) n += i1 - lo;
+    }
+    return n;
+}
+
+// One task of a build, with its precision rule: a quartet whose P bin and Q
+// bin upper bounds satisfy ub_P + ub_Q < log_cut is evaluated in FP32.
+struct TaskCtx {
+    BasisProdCache *bp;
+    int cp_ij, cp_kl, nbins;
+    const int *bins_ij, *bins_kl;
+    int li, lj, lk, ll, pij, pkl;
+    bool swap, diag;
+    double log_cut;      // -inf: FP64 only
+    const double *lq;    // host log q per pair (plan order) or NULL
+    double qflops;       // FLOPs of one quartet of this class
+};
+
+// log q upper bound of bin b of pair type cp (first pair of the bin in the
+// q-descending plan order); +inf for an empty bin or without log q
+static double bin_ub(const TaskCtx &c, int cp, const int *bins, int b) {
+    if (!c.lq || bins[b + 1] <= bins[b]) return 1e300;
+    return c.lq[c.bp->bas_pairs_locs[cp] + bins[b]];
+}
+
+static int task_init(TaskCtx &c, BasisProdCache *bp, const int *bins_ij, const int *bins_kl,
+                     int nbins, int cp_ij, int cp_kl, const double *lq, double dmax,
+                     double fp32_cut, int precision) {
+    c.bp = bp;
+    c.cp_ij = cp_ij;
+    c.cp_kl = cp_kl;
+    c.nbins = nbins;
+    c.bins_ij = bins_ij;
+    c.bins_kl = bins_kl;
     const ContractionProdType *tij = bp->cptype + cp_ij, *tkl = bp->cptype + cp_kl;
-    int li = tij->l_bra, lj = tij->l_ket, lk = tkl->l_bra, ll = tkl->l_ket;
-    if ((li + lj + lk + ll) / 2 + 1 >= 12) return 2;
-    if (li > BQC_LMAX || lj > BQC_LMAX || lk > BQC_LMAX || ll > BQC_LMAX || lj > li ||
-        ll > lk)
+    c.li = tij->l_bra;
+    c.lj = tij->l_ket;
+    c.lk = tkl->l_bra;
+    c.ll = tkl->l_ket;
+    if ((c.li + c.lj + c.lk + c.ll) / 2 + 1 >= 12) return 2;
+    if (c.li > BQC_LMAX || c.lj > BQC_LMAX || c.lk > BQC_LMAX || c.ll > BQC_LMAX ||
+        c.lj > c.li || c.ll > c.lk)
         return 1;
     if (nbins > BQC_MAX_BINS) return 3;
-    register_kernels();
-    TaskArgs ta;
-    memset(&ta, 0, sizeof(ta));
-    int pij = pair_id(li, lj), pkl = pair_id(lk, ll);
-    ta.swap = pij < pkl;
-    ta.diag = cp_ij == cp_kl;
-    const int *locs = bp->bas_pairs_locs, *plocs = bp->primitive_pairs_locs;
-    int cp_p = ta.swap ? cp_kl : cp_ij, cp_q = ta.swap ? cp_ij : cp_kl;
-    ta.bas_p0 = locs[cp_p];
-    ta.bas_q0 = locs[cp_q];
-    ta.prim_p0 = plocs[cp_p];
-    ta.prim_q0 = plocs[cp_q];
-    ta.nprim_p = bp->cptype[cp_p].nprim_12;
-    ta.nprim_q = bp->cptype[cp_q].nprim_12;
-    // work rectangles: kl pairs of bin b x ij pairs of bins [0, nbins-b)
-    int nb = 0;
-    ta.pre[0] = 0;
-    double nq = 0;
+    if (bins_ij[0] != 0 || bins_kl[0] != 0) return 1;
+    c.pij = pair_id(c.li, c.lj);
+    c.pkl = pair_id(c.lk, c.ll);
+    c.swap = c.pij < c.pkl;
+    c.diag = cp_ij == cp_kl;
+    c.lq = lq;
+    c.log_cut = (precision == 1 && lq && dmax > 0) ? log(fp32_cut / dmax) : -1e300;
+    c.qflops = quartet_flops(c.li, c.lj, c.lk, c.ll, tij->nprim_12 * tkl->nprim_12);
+    return 0;
+}
+
+static void record_launch(LaunchRec &rec, int cls, int kind, double flops, double nq,
+                          hipStream_t stream) {
+    rec.cls = cls;
+    rec.kind = kind;
+    rec.flops = flops;
+    rec.nquart = nq;
+    rec.e0 = prof_event();
+    rec.e1 = prof_event();
+    hipEventRecord(rec.e0, stream);
+}
+
+static int eval_cls(const TaskCtx &c) {
+    return c.swap ? c.lk * 1000 + c.ll * 100 + c.li * 10 + c.lj
+                  : c.li * 1000 + c.lj * 100 + c.lk * 10 + c.ll;
+}
+
+// Per-lane scratch (private segment) limit for a launch; the runtime aborts
+// the process when it cannot back a kernel's scratch, so oversized kernels
+// are refused here instead.
+#define BQC_MAX_SCRATCH_BYTES 32768
+static bool scratch_ok(const void *kern) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, kern) != hipSuccess) return false;
+    return attr.localSizeBytes <= BQC_MAX_SCRATCH_BYTES;
+}
+
+// Generic kernels (one lane per quartet): rectangle kl-bin b x ij in
+// [0, bins_ij[nbins-b]) split at the first FP32 ij bin.
+static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &pd,
+                           hipStream_t stream, double *flops) {
+    BasisProdCache *bp = c.bp;
+    const int nbins = c.nbins;
+    const int *bins_ij = c.bins_ij, *bins_kl = c.bins_kl;
+    TaskArgs ta[2];
+    double nq[2] = {0, 0};
+    for (int prec = 0; prec < 2; ++prec) {
+        TaskArgs &t = ta[prec];
+        memset(&t, 0, sizeof(t));
+        t.swap = c.swap;
+        t.diag = c.diag;
+        const int *locs = bp->bas_pairs_locs, *plocs = bp->primitive_pairs_locs;
+        int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
+        t.bas_p0 = locs[cp_p];
+        t.bas_q0 = locs[cp_q];
+        t.prim_p0 = plocs[cp_p];
+        t.prim_q0 = plocs[cp_q];
+        t.nprim_p = bp->cptype[cp_p].nprim_12;
+        t.nprim_q = bp->cptype[cp_q].nprim_12;
+        t.pre[0] = 0;
+    }
     for (int b = 0; b < nbins; ++b) {
-        int nkl = bins_kl[b + 1] - bins_kl[b];
-        int nij = bins_ij[nbins - b] - bins_ij[0];
-        if (nkl <= 0 || nij <= 0) continue;
-        ta.klb[nb] = bins_kl[b];
-        ta.nij[nb] = nij;
-        ta.pre[nb + 1] = ta.pre[nb] + (long long)nkl * nij;
-        if (ta.diag) {
-            for (int kl = bins_kl[b]; kl < bins_kl[b + 1]; ++kl) nq += nij > kl ? nij - kl : 0;
-        } else {
-            nq += (double)nkl * nij;
+        int k0 = bins_kl[b], k1 = bins_kl[b + 1];
+        int iend_bin = nbins - b;
+        if (k1 <= k0 || bins_ij[iend_bin] <= 0) continue;
+        double ubk = bin_ub(c, c.cp_kl, bins_kl, b);
+        int split = iend_bin;
+        for (int bi = 0; bi < iend_bin; ++bi)
+            if (bin_ub(c, c.cp_ij, bins_ij, bi) + ubk < c.log_cut) {
+                split = bi;
+                break;
+            }
+        int ranges[2][2] = {{0, bins_ij[split]}, {bins_ij[split], bins_ij[iend_bin]}};
+        for (int prec = 0; prec < 2; ++prec) {
+            int i0 = ranges[prec][0], i1 = ranges[prec][1];
+            if (i1 <= i0) continue;
+            TaskArgs &t = ta[prec];
+            int r = t.nbins;
+            t.klb[r] = k0;
+            t.ij0[r] = i0;
+            t.nij[r] = i1 - i0;
+            t.pre[r + 1] = t.pre[r] + (long long)(k1 - k0) * (i1 - i0);
+            t.nbins = r + 1;
+            nq[prec] += count_rect(i0, i1, k0, k1, c.diag);
         }
-        ++nb;
     }
-    ta.nbins = nb;
-    if (nb == 0) return 0;
-    if (bins_ij[0] != 0) return 1;
-    const double lflops = nq * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
-    if (flops) *flops += lflops;
-    JKKernel kern = ta.swap ? g_kernels[pkl][pij] : g_kernels[pij][pkl];
-    if (!kern) return 1;
-    long long ntot = ta.pre[nb];
-    long long nblk = (ntot + BQC_THREADS - 1) / BQC_THREADS;
-    LaunchRec rec;
-    if (g_prof) {
-        int a = ta.swap ? 1 : 0;
-        rec.cls = a ? lk * 1000 + ll * 100 + li * 10 + lj : li * 1000 + lj * 100 + lk * 10 + ll;
-        rec.kind = 0;
-        rec.flops = lflops;
-        rec.nquart = nq;
-        rec.e0 = prof_event();
-        rec.e1 = prof_event();
-        hipEventRecord(rec.e0, stream);
-    }
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, ta);
-    if (g_prof) {
-        hipEventRecord(rec.e1, stream);
-        g_recs.push_back(rec);
-    }
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) {
-        fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", li, lj, lk, ll,
-                hipGetErrorString(err));
-        return 1;
+    for (int prec = 0; prec < 2; ++prec) {
+        TaskArgs &t = ta[prec];
+        if (t.nbins == 0) continue;
+        const double lflops = nq[prec] * c.qflops;
+        flops[prec] += lflops;
+        JKKernel kern = c.swap ? g_kernels[prec][c.pkl][c.pij] : g_kernels[prec][c.pij][c.pkl];
+        if (!kern) return 1;
+        if (!scratch_ok((const void *)kern)) {
+            fprintf(stderr, "bqcjk: (%d%d|%d%d) kernel needs more scratch than allowed\n", c.li,
+                    c.lj, c.lk, c.ll);
+            return 1;
+        }
+        long long nblk = (t.pre[t.nbins] + BQC_THREADS - 1) / BQC_THREADS;
+        LaunchRec rec;
+        if (g_prof) record_launch(rec, eval_cls(c), 2 * prec, lflops, nq[prec], stream);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, t);
+        if (g_prof) {
+            hipEventRecord(rec.e1, stream);
+            g_recs.push_back(rec);
+        }
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) {
+            fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", c.li, c.lj, c.lk, c.ll,
+                    hipGetErrorString(err));
+            return 1;
+        }
     }
     return 0;
 }
@@ -183,7 +276,7 @@ struct RowLaunch {
     RowKernel kern;
     V2Task task;
     size_t off_perm_p, off_perm_q, off_items;
-    int cls;
+    int cls, prec;
     double flops, nquart;
 };
 
@@ -242,83 +335,102 @@ static void side_order(const int *bins, int nbins, const int *order, int npairs_
     }
 }
 
-// returns -1 if the task's class has no row kernel (caller falls back)
-static int plan_rows(BasisProdCache *bp, const int *bins_ij, const int *bins_kl, int nbins,
-                     int cp_ij, int cp_kl, const int *bra_order, std::vector<int> &stage,
+// Plans the row-kernel launches of one task (FP64 and FP32 item lists).
+// Returns -1 if the class has no row kernel (caller falls back).
+static int plan_rows(const TaskCtx &c, const int *bra_order, std::vector<int> &stage,
                      std::vector<RowLaunch> &rows, double *flops) {
-    const ContractionProdType *tij = bp->cptype + cp_ij, *tkl = bp->cptype + cp_kl;
-    int li = tij->l_bra, lj = tij->l_ket, lk = tkl->l_bra, ll = tkl->l_ket;
-    if (li > BQC_LMAX || lj > BQC_LMAX || lk > BQC_LMAX || ll > BQC_LMAX || lj > li || ll > lk)
-        return 1;
-    int pij = pair_id(li, lj), pkl = pair_id(lk, ll);
-    bool swap = pij < pkl;
-    RowKernel kern = swap ? g_rows[pkl][pij] : g_rows[pij][pkl];
-    if (!kern || !bra_order) return -1;
-    const int *pbins = swap ? bins_kl : bins_ij;
-    const int *qbins = swap ? bins_ij : bins_kl;
-    int cp_p = swap ? cp_kl : cp_ij, cp_q = swap ? cp_ij : cp_kl;
+    RowKernel k64 = c.swap ? g_rows[0][c.pkl][c.pij] : g_rows[0][c.pij][c.pkl];
+    RowKernel k32 = c.swap ? g_rows[1][c.pkl][c.pij] : g_rows[1][c.pij][c.pkl];
+    if (!k64 || !k32 || !bra_order) return -1;
+    if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32)) return 1;
+    BasisProdCache *bp = c.bp;
+    const int nbins = c.nbins;
+    const int *pbins = c.swap ? c.bins_kl : c.bins_ij;
+    const int *qbins = c.swap ? c.bins_ij : c.bins_kl;
+    int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
     const int *locs = bp->bas_pairs_locs;
     int np = pbins[nbins], nq = qbins[nbins];
-    if (pbins[0] != 0 || qbins[0] != 0) return 1;
-    // quartet count for the FLOP model
-    double nquart = 0;
-    for (int b = 0; b < nbins; ++b) {
-        int nkl = bins_kl[b + 1] - bins_kl[b];
-        int nij = bins_ij[nbins - b];
-        if (nkl <= 0 || nij <= 0) continue;
-        if (cp_ij == cp_kl) {
-            for (int kl = bins_kl[b]; kl < bins_kl[b + 1]; ++kl) nquart += nij > kl ? nij - kl : 0;
-        } else {
-            nquart += (double)nkl * nij;
+    // FLOPs split at bin granularity (total is exact)
+    double nquart[2] = {0, 0};
+    for (int bq = 0; bq < nbins; ++bq) {
+        int q0 = qbins[bq], q1 = qbins[bq + 1];
+        if (q1 <= q0) continue;
+        double ubq = bin_ub(c, cp_q, qbins, bq);
+        for (int bpn = 0; bpn < nbins - bq; ++bpn) {
+            int p0 = pbins[bpn], p1 = pbins[bpn + 1];
+            if (p1 <= p0) continue;
+            int prec = bin_ub(c, cp_p, pbins, bpn) + ubq < c.log_cut ? 1 : 0;
+            nquart[prec] += count_rect(p0, p1, q0, q1, c.diag);
         }
     }
-    const double lflops = nquart * quartet_flops(li, lj, lk, ll, tij->nprim_12 * tkl->nprim_12);
-    if (flops) *flops += lflops;
+    flops[0] += nquart[0] * c.qflops;
+    flops[1] += nquart[1] * c.qflops;
     if (np == 0 || nq == 0) return 0;
-    RowLaunch r;
-    memset(&r, 0, sizeof(r));
-    r.kern = kern;
-    r.cls = swap ? lk * 1000 + ll * 100 + li * 10 + lj : li * 1000 + lj * 100 + lk * 10 + ll;
-    r.flops = lflops;
-    r.nquart = nquart;
-    V2Task &tk = r.task;
-    tk.np = np;
-    tk.bas_p0 = locs[cp_p];
-    tk.bas_q0 = locs[cp_q];
-    tk.prim_p0 = bp->primitive_pairs_locs[cp_p];
-    tk.prim_q0 = bp->primitive_pairs_locs[cp_q];
-    tk.nprim_p = bp->cptype[cp_p].nprim_12;
-    tk.nprim_q = bp->cptype[cp_q].nprim_12;
-    tk.diag = cp_ij == cp_kl;
-    tk.nbins = nbins;
-    for (int b = 0; b <= nbins; ++b) {
-        tk.pbins[b] = pbins[b];
-        tk.qbins[b] = qbins[b];
+    RowLaunch r[2];
+    for (int prec = 0; prec < 2; ++prec) {
+        memset(&r[prec], 0, sizeof(RowLaunch));
+        r[prec].kern = prec ? k32 : k64;
+        r[prec].prec = prec;
+        r[prec].cls = eval_cls(c);
+        r[prec].flops = nquart[prec] * c.qflops;
+        r[prec].nquart = nquart[prec];
+        V2Task &tk = r[prec].task;
+        tk.np = np;
+        tk.bas_p0 = locs[cp_p];
+        tk.bas_q0 = locs[cp_q];
+        tk.prim_p0 = bp->primitive_pairs_locs[cp_p];
+        tk.prim_q0 = bp->primitive_pairs_locs[cp_q];
+        tk.nprim_p = bp->cptype[cp_p].nprim_12;
+        tk.nprim_q = bp->cptype[cp_q].nprim_12;
+        tk.diag = c.diag;
+        tk.nbins = nbins;
+        for (int b = 0; b <= nbins; ++b) {
+            tk.pbins[b] = pbins[b];
+            tk.qbins[b] = qbins[b];
+        }
     }
-    r.off_perm_p = stage.size();
+    size_t off_p = stage.size();
     side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], stage);
-    if (cp_p == cp_q) {
-        r.off_perm_q = r.off_perm_p;
-    } else {
-        r.off_perm_q = stage.size();
+    size_t off_q = off_p;
+    if (cp_p != cp_q) {
+        off_q = stage.size();
         side_order(qbins, nbins, bra_order + locs[cp_q], locs[cp_q + 1] - locs[cp_q], stage);
     }
-    if (stage.size() & 1) stage.push_back(0);  // int2 alignment
-    r.off_items = stage.size();
+    std::vector<int> items[2];
     int nchunk = (np + 63) / 64;
-    int nitems = 0;
-    for (int c = 0; c < nchunk; ++c) {
-        int first = c * 64;
+    for (int ch = 0; ch < nchunk; ++ch) {
+        int first = ch * 64;
         int s = (int)(std::upper_bound(pbins + 1, pbins + nbins + 1, first) - (pbins + 1));
         int qmax = qbins[nbins - s];
-        for (int q0 = 0; q0 < qmax; q0 += BQC_V2_CK) {
-            stage.push_back(c);
-            stage.push_back(q0);
-            ++nitems;
-        }
+        double ubp = bin_ub(c, cp_p, pbins, s);
+        int split = nbins - s;
+        for (int bq = 0; bq < nbins - s; ++bq)
+            if (ubp + bin_ub(c, cp_q, qbins, bq) < c.log_cut) {
+                split = bq;
+                break;
+            }
+        int qsplit = qbins[split];
+        int lim[2][2] = {{0, qsplit < qmax ? qsplit : qmax}, {qsplit, qmax}};
+        for (int prec = 0; prec < 2; ++prec)
+            for (int q0 = lim[prec][0]; q0 < lim[prec][1]; q0 += BQC_V2_CK) {
+                int q1 = q0 + BQC_V2_CK < lim[prec][1] ? q0 + BQC_V2_CK : lim[prec][1];
+                items[prec].push_back(ch);
+                items[prec].push_back(q0);
+                items[prec].push_back(q1);
+                items[prec].push_back(0);
+            }
     }
-    tk.nitems = nitems;
-    if (nitems) rows.push_back(r);
+    for (int prec = 0; prec < 2; ++prec) {
+        int nitems = (int)items[prec].size() / 4;
+        if (!nitems) continue;
+        while (stage.size() & 3) stage.push_back(0);  // int4 alignment
+        r[prec].off_perm_p = off_p;
+        r[prec].off_perm_q = off_q;
+        r[prec].off_items = stage.size();
+        stage.insert(stage.end(), items[prec].begin(), items[prec].end());
+        r[prec].task.nitems = nitems;
+        rows.push_back(r[prec]);
+    }
     return 0;
 }
 
@@ -327,35 +439,38 @@ extern "C" {
 int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                  int n_dm, int *bins_locs_ij, int *bins_locs_kl, int nbins, int cp_ij_id,
                  int cp_kl_id, hipStream_t stream) {
+    register_kernels();
     JKArgs jk = {vj, vk, dm, nao, n_dm};
     PairData pd = make_pair_data(bpcache);
-    return launch_task(bpcache, jk, pd, bins_locs_ij, bins_locs_kl, nbins, cp_ij_id,
-                       cp_kl_id, stream, nullptr);
+    TaskCtx c;
+    int err = task_init(c, bpcache, bins_locs_ij, bins_locs_kl, nbins, cp_ij_id, cp_kl_id,
+                        nullptr, 0, 0, 0);
+    if (err) return err;
+    double fl[2] = {0, 0};
+    return launch_quartets(c, jk, pd, stream, fl);
 }
 
 int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                 int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
                 const int *bra_order, int precision, const double *log_q,
                 const double *task_dmax, double fp32_cut, hipStream_t stream) {
-    (void)log_q;
-    (void)task_dmax;
-    (void)fp32_cut;
-    if (precision != 0) return 1;
+    if (precision != 0 && precision != 1) return 1;
     if (nbins > BQC_MAX_BINS) return 3;
     register_kernels();
     JKArgs jk = {vj, vk, dm, nao, n_dm};
     PairData pd = make_pair_data(bpcache);
-    double flops = 0;
+    double flops[2] = {0, 0};
     std::vector<RowLaunch> rows;
     std::vector<int> stage;
     for (int t = 0; t < ntask; ++t) {
         const int *bi = task_bins + (size_t)t * 2 * (nbins + 1);
         const int *bk = bi + nbins + 1;
-        int cp_ij = task_cp[2 * t], cp_kl = task_cp[2 * t + 1];
-        int err = plan_rows(bpcache, bi, bk, nbins, cp_ij, cp_kl, bra_order, stage, rows, &flops);
-        if (err < 0) {
-            err = launch_task(bpcache, jk, pd, bi, bk, nbins, cp_ij, cp_kl, stream, &flops);
-        }
+        TaskCtx c;
+        int err = task_init(c, bpcache, bi, bk, nbins, task_cp[2 * t], task_cp[2 * t + 1], log_q,
+                            task_dmax ? task_dmax[t] : 0, fp32_cut, precision);
+        if (err) return err;
+        err = plan_rows(c, bra_order, stage, rows, flops);
+        if (err < 0) err = launch_quartets(c, jk, pd, stream, flops);
         if (err > 0) return err;
     }
     if (!rows.empty()) {
@@ -364,18 +479,10 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
         for (RowLaunch &r : rows) {
             r.task.perm_p = g_dws + r.off_perm_p;
             r.task.perm_q = g_dws + r.off_perm_q;
-            r.task.items = (const int2 *)(g_dws + r.off_items);
+            r.task.items = (const int4 *)(g_dws + r.off_items);
             unsigned nblk = (unsigned)((r.task.nitems + 3) / 4);
             LaunchRec rec;
-            if (g_prof) {
-                rec.cls = r.cls;
-                rec.kind = 1;
-                rec.flops = r.flops;
-                rec.nquart = r.nquart;
-                rec.e0 = prof_event();
-                rec.e1 = prof_event();
-                hipEventRecord(rec.e0, stream);
-            }
+            if (g_prof) record_launch(rec, r.cls, 1 + 2 * r.prec, r.flops, r.nquart, stream);
             hipLaunchKernelGGL(r.kern, dim3(nblk), dim3(256), 0, stream, jk, pd, r.task);
             if (g_prof) {
                 hipEventRecord(rec.e1, stream);
@@ -388,8 +495,8 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
             }
         }
     }
-    g_last_flops[0] = flops;
-    g_last_flops[1] = 0;
+    g_last_flops[0] = flops[0];
+    g_last_flops[1] = flops[1];
     return 0;
 }
 

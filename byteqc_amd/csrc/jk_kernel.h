@@ -41,6 +41,7 @@ struct TaskArgs {
     int swap;              // p side is the task's kl list
     int nbins;
     int klb[BQC_MAX_BINS + 1];
+    int ij0[BQC_MAX_BINS];
     int nij[BQC_MAX_BINS];
     long long pre[BQC_MAX_BINS + 1];
 };
@@ -64,15 +65,20 @@ __device__ __forceinline__ void atomic_add(double *p, double v) {
 }
 
 // Integral block of (p | q): p = shells (a, b) of the "bra" pair, q = (c, d).
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void eri_block(double *__restrict__ gout, const PairData &pd,
+// T is the arithmetic type of the recurrence and of the contracted block;
+// roots, weights, prefactors and recurrence coefficients are always FP64.
+// XN consecutive Cartesian components of shell a starting at x0 are
+// evaluated (XN = NFI: the whole block); gout holds XN*NFJ*NFK*NFL values.
+template <int LA, int LB, int LC, int LD, typename T, int XN = Cls<LA, LB, LC, LD>::NFI>
+__device__ __forceinline__ void eri_block(T *__restrict__ gout, const PairData &pd,
                                           int pp0, int np_p, int qq0, int np_q, double norm,
                                           const double *ra, const double *rb,
-                                          const double *rc, const double *rd) {
+                                          const double *rc, const double *rd, int x0 = 0) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NR = C::NR;
+    constexpr int NG = XN * C::NFJ * C::NFK * C::NFL;
 #pragma unroll C::UN
-    for (int n = 0; n < C::NF; ++n) gout[n] = 0.0;
+    for (int n = 0; n < NG; ++n) gout[n] = (T)0;
     const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
     const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
     for (int ip = 0; ip < np_p; ++ip) {
@@ -100,19 +106,20 @@ __device__ __forceinline__ void eri_block(double *__restrict__ gout, const PairD
                 const double b01 = b00 + tmp4 * aij;
                 const double tmp2 = 2.0 * b00 * akl;
                 const double tmp3 = 2.0 * b00 * aij;
-                double Ix[C::GSZ], Iy[C::GSZ], Iz[C::GSZ];
-                rys2d<LA, LB, LC, LD, double>(Ix, norm, (P[0] - ra[0]) - tmp2 * pq[0],
-                                              (Q[0] - rc[0]) + tmp3 * pq[0], b10, b01, b00,
-                                              ab[0], cd[0]);
-                rys2d<LA, LB, LC, LD, double>(Iy, fac, (P[1] - ra[1]) - tmp2 * pq[1],
-                                              (Q[1] - rc[1]) + tmp3 * pq[1], b10, b01, b00,
-                                              ab[1], cd[1]);
-                rys2d<LA, LB, LC, LD, double>(Iz, w[r], (P[2] - ra[2]) - tmp2 * pq[2],
-                                              (Q[2] - rc[2]) + tmp3 * pq[2], b10, b01, b00,
-                                              ab[2], cd[2]);
+                T Ix[C::GSZ], Iy[C::GSZ], Iz[C::GSZ];
+                rys2d<LA, LB, LC, LD, T>(Ix, (T)norm, (T)((P[0] - ra[0]) - tmp2 * pq[0]),
+                                         (T)((Q[0] - rc[0]) + tmp3 * pq[0]), (T)b10, (T)b01,
+                                         (T)b00, (T)ab[0], (T)cd[0]);
+                rys2d<LA, LB, LC, LD, T>(Iy, (T)fac, (T)((P[1] - ra[1]) - tmp2 * pq[1]),
+                                         (T)((Q[1] - rc[1]) + tmp3 * pq[1]), (T)b10, (T)b01,
+                                         (T)b00, (T)ab[1], (T)cd[1]);
+                rys2d<LA, LB, LC, LD, T>(Iz, (T)w[r], (T)((P[2] - ra[2]) - tmp2 * pq[2]),
+                                         (T)((Q[2] - rc[2]) + tmp3 * pq[2]), (T)b10, (T)b01,
+                                         (T)b00, (T)ab[2], (T)cd[2]);
 #pragma unroll C::UO
-                for (int x = 0; x < C::NFI; ++x) {
-                    const int ax = cart_x(LA, x), ay = cart_y(LA, x), az = cart_z(LA, x);
+                for (int x = 0; x < XN; ++x) {
+                    const int ax = cart_x(LA, x0 + x), ay = cart_y(LA, x0 + x),
+                              az = cart_z(LA, x0 + x);
 #pragma unroll C::UO
                     for (int y = 0; y < C::NFJ; ++y) {
                         const int bx = cart_x(LB, y), by = cart_y(LB, y), bz = cart_z(LB, y);
@@ -135,10 +142,12 @@ __device__ __forceinline__ void eri_block(double *__restrict__ gout, const PairD
 }
 
 // J/K digestion of one integral block (a b | c d) with AO offsets a0..d0.
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void digest(const double *__restrict__ gout, const JKArgs &jk,
-                                       int a0, int b0, int c0, int d0) {
+// Digests the XN components of shell a starting at x0 (see eri_block).
+template <int LA, int LB, int LC, int LD, typename T, int XN = Cls<LA, LB, LC, LD>::NFI>
+__device__ __forceinline__ void digest(const T *__restrict__ gout, const JKArgs &jk,
+                                       int a0, int b0, int c0, int d0, int x0 = 0) {
     using C = Cls<LA, LB, LC, LD>;
+    a0 += x0;
     const int nao = jk.nao;
     const size_t nao2 = (size_t)nao * nao;
     for (int idm = 0; idm < jk.n_dm; ++idm) {
@@ -146,7 +155,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
         if (jk.vj) {
             double *V = jk.vj + idm * nao2;
 #pragma unroll C::UO
-            for (int ia = 0; ia < C::NFI; ++ia)
+            for (int ia = 0; ia < XN; ++ia)
 #pragma unroll
                 for (int ib = 0; ib < C::NFJ; ++ib) {
                     double s = 0.0;
@@ -154,7 +163,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                     for (int ic = 0; ic < C::NFK; ++ic)
 #pragma unroll
                         for (int id = 0; id < C::NFL; ++id)
-                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                            s += (double)gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
                                  D[(c0 + ic) * nao + d0 + id];
                     atomic_add(V + (size_t)(a0 + ia) * nao + b0 + ib, s);
                 }
@@ -164,10 +173,10 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                 for (int id = 0; id < C::NFL; ++id) {
                     double s = 0.0;
 #pragma unroll C::UO
-                    for (int ia = 0; ia < C::NFI; ++ia)
+                    for (int ia = 0; ia < XN; ++ia)
 #pragma unroll
                         for (int ib = 0; ib < C::NFJ; ++ib)
-                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                            s += (double)gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
                                  D[(a0 + ia) * nao + b0 + ib];
                     atomic_add(V + (size_t)(c0 + ic) * nao + d0 + id, s);
                 }
@@ -176,7 +185,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
             double *V = jk.vk + idm * nao2;
             // V[a,c] += g D[b,d]; V[a,d] += g D[b,c]
 #pragma unroll C::UO
-            for (int ia = 0; ia < C::NFI; ++ia) {
+            for (int ia = 0; ia < XN; ++ia) {
 #pragma unroll
                 for (int ic = 0; ic < C::NFK; ++ic) {
                     double s = 0.0;
@@ -184,7 +193,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                     for (int ib = 0; ib < C::NFJ; ++ib)
 #pragma unroll
                         for (int id = 0; id < C::NFL; ++id)
-                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                            s += (double)gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
                                  D[(b0 + ib) * nao + d0 + id];
                     atomic_add(V + (size_t)(a0 + ia) * nao + c0 + ic, s);
                 }
@@ -195,7 +204,7 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                     for (int ib = 0; ib < C::NFJ; ++ib)
 #pragma unroll
                         for (int ic = 0; ic < C::NFK; ++ic)
-                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                            s += (double)gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
                                  D[(b0 + ib) * nao + c0 + ic];
                     atomic_add(V + (size_t)(a0 + ia) * nao + d0 + id, s);
                 }
@@ -207,10 +216,10 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                 for (int ic = 0; ic < C::NFK; ++ic) {
                     double s = 0.0;
 #pragma unroll C::UO
-                    for (int ia = 0; ia < C::NFI; ++ia)
+                    for (int ia = 0; ia < XN; ++ia)
 #pragma unroll
                         for (int id = 0; id < C::NFL; ++id)
-                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                            s += (double)gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
                                  D[(a0 + ia) * nao + d0 + id];
                     atomic_add(V + (size_t)(b0 + ib) * nao + c0 + ic, s);
                 }
@@ -218,10 +227,10 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
                 for (int id = 0; id < C::NFL; ++id) {
                     double s = 0.0;
 #pragma unroll C::UO
-                    for (int ia = 0; ia < C::NFI; ++ia)
+                    for (int ia = 0; ia < XN; ++ia)
 #pragma unroll
                         for (int ic = 0; ic < C::NFK; ++ic)
-                            s += gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
+                            s += (double)gout[((ia * C::NFJ + ib) * C::NFK + ic) * C::NFL + id] *
                                  D[(a0 + ia) * nao + c0 + ic];
                     atomic_add(V + (size_t)(b0 + ib) * nao + d0 + id, s);
                 }
@@ -230,8 +239,9 @@ __device__ __forceinline__ void digest(const double *__restrict__ gout, const JK
     }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T>
 __global__ void __launch_bounds__(BQC_THREADS)
+    __attribute__((amdgpu_waves_per_eu(1, Cls<LA, LB, LC, LD>::BIG ? 1 : 8)))
     jk_quartets(JKArgs jk, PairData pd, TaskArgs ta) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ta.pre[ta.nbins]) return;
@@ -239,7 +249,7 @@ __global__ void __launch_bounds__(BQC_THREADS)
     while (t >= ta.pre[b + 1]) ++b;
     const long long local = t - ta.pre[b];
     const int kl = ta.klb[b] + (int)(local / ta.nij[b]);
-    const int ij = (int)(local % ta.nij[b]);
+    const int ij = ta.ij0[b] + (int)(local % ta.nij[b]);
     // p side = bra pair as evaluated; the task's (ij, kl) roles swap when the
     // ket class is the larger one
     const int bas_ij = (ta.swap ? ta.bas_q0 : ta.bas_p0) + ij;
@@ -256,20 +266,35 @@ __global__ void __launch_bounds__(BQC_THREADS)
     const double rb[3] = {pd.bas_x[sb], pd.bas_y[sb], pd.bas_z[sb]};
     const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
     const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
-    double gout[Cls<LA, LB, LC, LD>::NF];
-    eri_block<LA, LB, LC, LD>(gout, pd, pp0, ta.nprim_p, qq0, ta.nprim_q, norm, ra, rb, rc, rd);
-    digest<LA, LB, LC, LD>(gout, jk, pd.ao_loc[sa], pd.ao_loc[sb], pd.ao_loc[sc],
-                           pd.ao_loc[sd]);
+    using C = Cls<LA, LB, LC, LD>;
+    if constexpr (C::BIG) {
+        // one Cartesian component of shell a at a time keeps the block small
+        for (int x0 = 0; x0 < C::NFI; ++x0) {
+            T gout[C::NF / C::NFI];
+            eri_block<LA, LB, LC, LD, T, 1>(gout, pd, pp0, ta.nprim_p, qq0, ta.nprim_q, norm, ra,
+                                            rb, rc, rd, x0);
+            digest<LA, LB, LC, LD, T, 1>(gout, jk, pd.ao_loc[sa], pd.ao_loc[sb], pd.ao_loc[sc],
+                                         pd.ao_loc[sd], x0);
+        }
+    } else {
+        T gout[C::NF];
+        eri_block<LA, LB, LC, LD, T>(gout, pd, pp0, ta.nprim_p, qq0, ta.nprim_q, norm, ra, rb, rc,
+                                     rd);
+        digest<LA, LB, LC, LD, T>(gout, jk, pd.ao_loc[sa], pd.ao_loc[sb], pd.ao_loc[sc],
+                                  pd.ao_loc[sd]);
+    }
 }
 
 
 typedef void (*JKKernel)(JKArgs, PairData, TaskArgs);
 #define BQC_LMAX 3
 #define BQC_NPAIRID ((BQC_LMAX + 1) * (BQC_LMAX + 2) / 2)
-typedef JKKernel JKKernelTable[BQC_NPAIRID][BQC_NPAIRID];
+// [precision: 0 FP64, 1 FP32][bra class][ket class]
+typedef JKKernel JKKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
 
 template <int LA, int LB, int LC, int LD>
 static void reg_kernel(JKKernelTable &tab) {
-    tab[LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_quartets<LA, LB, LC, LD>;
+    tab[0][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_quartets<LA, LB, LC, LD, double>;
+    tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_quartets<LA, LB, LC, LD, float>;
 }
 

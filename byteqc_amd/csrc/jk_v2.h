@@ -25,7 +25,7 @@
 struct V2Task {
     const int *perm_p;  // list position -> plan position, P side
     const int *perm_q;
-    const int2 *items;  // (chunk, q0)
+    const int4 *items;  // (chunk, q0, q1, -)
     int nitems;
     int np;             // P pairs in bins
     int bas_p0, bas_q0, prim_p0, prim_q0, nprim_p, nprim_q;
@@ -52,16 +52,17 @@ __device__ __forceinline__ double run_sum(double v, int lane, int run_end) {
     return v;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T>
 __global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NA = C::NFI, NB = C::NFJ, NC = C::NFK, ND = C::NFL;
     const int lane = threadIdx.x & 63;
     const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (item >= tk.nitems) return;
-    const int2 it = tk.items[item];
+    const int4 it = tk.items[item];
     const int chunk = __builtin_amdgcn_readfirstlane(it.x);
     const int q0 = __builtin_amdgcn_readfirstlane(it.y);
+    const int qe = __builtin_amdgcn_readfirstlane(it.z);
     const int pidx = chunk * 64 + lane;
     const bool pvalid = pidx < tk.np;
     // bin of this P pair -> end of its Q prefix
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk
         qlim = tk.qbins[tk.nbins - s];
         ppos = tk.perm_p[pidx];
     }
-    const int qend_item = min(q0 + BQC_V2_CK, tk.qbins[tk.nbins]);
+    const int qend_item = min(qe, tk.qbins[tk.nbins]);
     int qlim_max = qlim;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) qlim_max = max(qlim_max, __shfl_xor(qlim_max, d, 64));
@@ -129,17 +130,17 @@ __global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk
                 active = active && ppos >= qpos;
                 if (ppos == qpos) norm *= 0.5;
             }
-            double g[C::NF];
+            T g[C::NF];
             if (active) {
                 const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
                 const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
-                eri_block<LA, LB, LC, LD>(g, pd, pp0, tk.nprim_p, tk.prim_q0 + qpos * tk.nprim_q,
+                eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p, tk.prim_q0 + qpos * tk.nprim_q,
                                           tk.nprim_q, norm, ra, rb, rc, rd);
             } else {
 #pragma unroll
-                for (int n = 0; n < C::NF; ++n) g[n] = 0.0;
+                for (int n = 0; n < C::NF; ++n) g[n] = (T)0;
             }
-#define G(x, y, z, w) g[(((x) * NB + (y)) * NC + (z)) * ND + (w)]
+#define G(x, y, z, w) ((double)g[(((x) * NB + (y)) * NC + (z)) * ND + (w)])
             if (VJ) {
                 // J(a,b) += g D(c,d)
 #pragma unroll
@@ -264,11 +265,12 @@ __global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk
 }
 
 typedef void (*RowKernel)(JKArgs, PairData, V2Task);
-typedef RowKernel RowKernelTable[BQC_NPAIRID][BQC_NPAIRID];
+typedef RowKernel RowKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
 
 template <int LA, int LB, int LC, int LD>
 static void reg_row(RowKernelTable &tab) {
-    tab[LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD>;
+    tab[0][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, double>;
+    tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, float>;
 }
 
 // Classes digested with the wavefront-stationary kernel (register budget:

@@ -299,9 +299,9 @@ class _VHFOpt:
             numpy.vstack([numpy.hstack(bra), numpy.hstack(ket)]).astype(numpy.int32))
         self.bas_pairs_locs = numpy.append(0, numpy.cumsum([len(x) for x in bra])).astype(
             numpy.int32)
-        # per pair type: pair positions ordered by first shell (stable)
+        # per pair type: pair positions ordered by first shell, then second
         self.bra_order = numpy.ascontiguousarray(numpy.concatenate(
-            [numpy.argsort(b, kind='stable') for b in bra] + [numpy.zeros(0, int)]).astype(
+            [numpy.lexsort((k, b)) for b, k in zip(bra, ket)] + [numpy.zeros(0, int)]).astype(
                 numpy.int32))
         self._init_bpcache(0.5 if not diag_block_with_triu else 1., dev)
         self.build_time = time.perf_counter() - t0
@@ -327,8 +327,10 @@ class _VHFOpt:
             bas_coords=torch.as_tensor(self._bas_coords, device=dev),
             aexyz=torch.as_tensor(aexyz.copy(), device=dev),
             pair2shls=torch.as_tensor(self.bas_pair2shls, device=dev),
-            log_q=torch.as_tensor(numpy.concatenate(self.log_qs + [numpy.zeros(1)]), device=dev),
         )
+        # log q of every pair in plan order (host; bin upper bounds for the
+        # mixed-precision split)
+        self.log_q = numpy.ascontiguousarray(numpy.concatenate(self.log_qs + [numpy.zeros(1)]))
         self.cp_nprim = numpy.array([bp.contents.cptype[i].nprim_12 for i in range(ncp)])
         bpg = ctypes.POINTER(BasisProdCache)()
         d = self._dev
@@ -434,7 +436,7 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None,
     if len(task_cp):
         err = lib.BQCbuild_jk(vhfopt.bpcache, _ptr(vj), _ptr(vk), _ptr(dmc), nao, n_dm,
                               len(task_cp), _ptr(task_cp), _ptr(task_bins), nbins,
-                              _ptr(vhfopt.bra_order), PRECISION[precision], _ptr(vhfopt._dev['log_q']),
+                              _ptr(vhfopt.bra_order), PRECISION[precision], _ptr(vhfopt.log_q),
                               _ptr(task_dmax), fp32_cut, s)
         if err:
             bad = []

@@ -103,10 +103,11 @@ void BQCdel_basis_prod_clone(BasisProdCache **pbp);
  * the pair list ordered by first shell (stable); it lets the kernels walk
  * each Schwarz bin grouped by first shell.  Builds that share a library
  * must be ordered on one stream (they share a device work list).  precision: 0 = FP64
- * throughout; 1 = mixed (quartets whose bound q_ij q_kl dmax falls below
- * fp32_cut are evaluated in FP32 after FP64 Rys roots; accumulation FP64).
- * q_ij for the mixed rule comes from log_q (device, one per pair, same order
- * as the pair lists) and dmax is the task's density bound (task_dmax, host).
+ * throughout; 1 = mixed: a quartet is evaluated in FP32 (Rys roots and
+ * weights, prefactors and all J/K accumulation stay FP64) when the upper
+ * bound q_P q_Q dmax of its Schwarz bins, taken over the 64 pairs sharing a
+ * wavefront, falls below fp32_cut.  log_q (host, one per pair, plan order)
+ * gives log q; task_dmax (host, one per task) the task's density bound.
  * Returns 0 ok, 1 launch error / unsupported class, 2 nroots >= 12,
  * 3 too many bins. */
 int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,

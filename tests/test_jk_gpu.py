@@ -113,3 +113,24 @@ def test_water_dimer_cc_pvtz_matches_oracle():
     jref, kref = orc.get_jk(mol, dm)
     dj, dk = _report('2 H2O cc-pVTZ', j, k, jref, kref)
     assert dj < TOL and dk < TOL
+
+
+def test_mixed_precision_c1_energy_bar():
+    """Mixed precision (FP32 for quartets with bound < 1e-7, FP64 roots and
+    accumulators) against the FP64 oracle fixture: |dE| <= 1e-8 Eh with
+    E = 1/2 Tr[D (J - K/2)], and the J/K deviations stated."""
+    _torch_gpu()
+    from byteqc_amd.scf import hf
+    from byteqc_amd.systems import workload
+    mol, dm, _ = workload('c1')
+    gold = numpy.load(os.path.join(GOLD, 'jk_c1.npz'))
+    j, k = hf.get_jk(mol, dm, precision='mixed', fp32_cut=1e-7)
+    fl = hf.last_build_flops()
+    e = 0.5 * numpy.einsum('ij,ij', dm, j - 0.5 * k)
+    eref = 0.5 * numpy.einsum('ij,ij', dm, gold['vj'] - 0.5 * gold['vk'])
+    dj, dk = _report('C1 mixed', j, k, gold['vj'], gold['vk'])
+    print(f'C1 mixed: |dE| {abs(e - eref):.3e} Eh, FP32 share of FLOPs {fl[1] / fl.sum():.3f}',
+          flush=True)
+    assert fl[1] > 0
+    assert abs(e - eref) <= 1e-8
+    assert dj <= 1e-8 and dk <= 1e-8
