@@ -27,6 +27,8 @@ _SIGNATURES = {
                             c_double, c_void_p]),
     'BQCq_cond': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int,
                           c_void_p, c_void_p]),
+    'BQCint1e': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                         c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     'BQCsymmetrize_jk': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
     'BQClast_build_flops': (None, [c_void_p]),
     'BQCprofile_enable': (None, [c_int]),

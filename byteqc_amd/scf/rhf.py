@@ -1,0 +1,169 @@
+"""Restricted Hartree-Fock on one GPU around the J/K engine.
+
+The reference runs PySCF's SCF loop with ``get_jk`` and ``_eigh`` replaced by
+its GPU versions (cuobc/scf/hf.py:253-274).  This driver keeps every SCF
+quantity on the device (S, H, X, D, F, DIIS history) and advances one
+iteration per ``step()`` call, so a request loop can interleave many SCF runs.
+
+Iteration: D = 2 C_occ C_occ^T; J, K = get_jk(D); F = H + J - K/2;
+E = 1/2 Tr[D (H + F)] + E_nuc; Pulay DIIS on e = X^T (F D S - S D F) X;
+F' = X^T F X diagonalised (X = S^-1/2, canonical, small eigenvalues dropped).
+Converged when |dE| < conv_tol and max|e| < conv_tol_grad.
+
+``precision='mixed_then_fp64'`` builds J/K in mixed precision while
+|dE| >= switch_de and in FP64 afterwards (and for the final energy).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy
+import torch
+
+from byteqc_amd.lib import load_library
+from byteqc_amd.scf import hf
+
+
+def int1e(opt, stream=None):
+    """(S, T, V) in the spherical contracted AO basis of the plan's molecule,
+    as device tensors."""
+    lib = load_library()
+    pm = opt.mol
+    dev = hf._device()
+    nao_c = int(pm.ao_loc_nr()[-1])
+    atm = torch.as_tensor(pm._atm, device=dev)
+    bas = torch.as_tensor(pm._bas, device=dev)
+    env = torch.as_tensor(pm._env, device=dev)
+    aol = opt._dev['ao_loc']
+    mats = [torch.zeros((nao_c, nao_c), dtype=torch.float64, device=dev) for _ in range(3)]
+    grp_l = numpy.ascontiguousarray(opt.uniq_l_ctr[:, 0], dtype=numpy.int32)
+    err = lib.BQCint1e(*(hf._ptr(m) for m in mats), hf._ptr(atm), pm.natm, hf._ptr(bas),
+                       hf._ptr(env), hf._ptr(aol), nao_c, hf._ptr(opt.l_ctr_offsets),
+                       len(grp_l), hf._ptr(grp_l), hf._stream_ptr(stream))
+    if err:
+        raise RuntimeError('one-electron integral kernel failed')
+    c = opt.coeff_dev
+    return tuple(c.T @ m @ c for m in mats)
+
+
+class RHF:
+    def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=80, diis_space=8,
+                 precision='fp64', switch_de=1e-5, fp32_cut=1e-7, lindep=1e-9, vhfopt=None,
+                 stream=None):
+        self.mol = mol
+        self.conv_tol = conv_tol
+        self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else conv_tol ** 0.5
+        self.max_cycle = max_cycle
+        self.diis_space = diis_space
+        self.precision = precision
+        self.switch_de = switch_de
+        self.fp32_cut = fp32_cut
+        self.lindep = lindep
+        self.stream = stream
+        self._opt = vhfopt
+        self.converged = False
+        self.e_tot = None
+        self.cycles = 0
+        self.history = []
+        self.jk_builds = {'fp64': 0, 'mixed': 0}
+
+    # -- set-up --------------------------------------------------------------
+    def build(self):
+        t0 = time.perf_counter()
+        if self._opt is None:
+            self._opt = hf._VHFOpt(self.mol).build()
+        s, t, v = int1e(self._opt, self.stream)
+        self.S, self.H = s, t + v
+        e, u = torch.linalg.eigh(s)
+        keep = e > self.lindep
+        self.X = u[:, keep] / torch.sqrt(e[keep])
+        self.e_nuc = float(self.mol.energy_nuc())
+        self.nocc = self.mol.nelectron // 2
+        self._f_hist, self._e_hist = [], []
+        self.mo_coeff = self._diag(self.H)
+        self.dm = self._make_dm(self.mo_coeff)
+        self.e_prev = None
+        self.build_time = time.perf_counter() - t0
+        return self
+
+    def _diag(self, f):
+        fo = self.X.T @ f @ self.X
+        self.mo_energy, c = torch.linalg.eigh(fo)
+        return self.X @ c
+
+    def _make_dm(self, c):
+        co = c[:, :self.nocc]
+        return 2.0 * co @ co.T
+
+    def _jk_precision(self):
+        if self.precision == 'fp64':
+            return 'fp64'
+        if self.precision == 'mixed':
+            return 'mixed'
+        # mixed_then_fp64
+        if self.e_prev is not None and self.history and abs(self.history[-1]['dE']) < self.switch_de:
+            return 'fp64'
+        return 'mixed'
+
+    # -- one iteration ---------------------------------------------------------
+    def step(self):
+        """One SCF iteration; returns True once converged."""
+        prec = self._jk_precision()
+        j, k = hf.get_jk(self.mol, self.dm, vhfopt=self._opt, return_device=True,
+                         precision=prec, fp32_cut=self.fp32_cut, stream=self.stream)
+        self.jk_builds[prec] += 1
+        f = self.H + j - 0.5 * k
+        e_elec = 0.5 * float(torch.sum(self.dm * (self.H + f)))
+        e_tot = e_elec + self.e_nuc
+        sdf = self.S @ self.dm @ f
+        err = self.X.T @ (sdf.T - sdf) @ self.X
+        err_max = float(err.abs().max())
+        de = e_tot - self.e_prev if self.e_prev is not None else e_tot
+        self.history.append({'e': e_tot, 'dE': de, 'err': err_max, 'precision': prec})
+        self.cycles += 1
+        self.e_tot = e_tot
+        conv = (self.e_prev is not None and abs(de) < self.conv_tol
+                and err_max < self.conv_tol_grad and prec == 'fp64') or \
+               (self.precision == 'mixed' and self.e_prev is not None
+                and abs(de) < self.conv_tol and err_max < self.conv_tol_grad)
+        self.e_prev = e_tot
+        if conv:
+            self.converged = True
+            return True
+        f = self._diis(f, err)
+        self.mo_coeff = self._diag(f)
+        self.dm = self._make_dm(self.mo_coeff)
+        return False
+
+    def _diis(self, f, err):
+        self._f_hist.append(f)
+        self._e_hist.append(err)
+        if len(self._f_hist) > self.diis_space:
+            self._f_hist.pop(0)
+            self._e_hist.pop(0)
+        n = len(self._f_hist)
+        if n < 2:
+            return f
+        b = torch.empty((n + 1, n + 1), dtype=torch.float64, device=f.device)
+        e = torch.stack(self._e_hist).reshape(n, -1)
+        b[:n, :n] = e @ e.T
+        b[n, :n] = b[:n, n] = -1.0
+        b[n, n] = 0.0
+        rhs = torch.zeros(n + 1, dtype=torch.float64, device=f.device)
+        rhs[n] = -1.0
+        c = torch.linalg.lstsq(b.cpu(), rhs.cpu().unsqueeze(1)).solution.squeeze(1).to(f.device)
+        return torch.einsum('i,ijk->jk', c[:n], torch.stack(self._f_hist))
+
+    def kernel(self):
+        if not hasattr(self, 'S'):
+            self.build()
+        for _ in range(self.max_cycle):
+            if self.step():
+                break
+        return self.e_tot
+
+
+def scf_energy(mol, **kwargs):
+    mf = RHF(mol, **kwargs)
+    return mf.kernel(), mf

@@ -122,6 +122,16 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
 int BQCq_cond(double *q, const int *atm, const int *bas, const double *env, int nbas,
               const int *grp_offsets, int ngroups, const int *grp_l, hipStream_t stream);
 
+/* One-electron integrals S, T, V (nuclear attraction of all atoms in atm)
+ * on the GPU, written as full symmetric nao x nao Cartesian matrices over the
+ * shells of bas (device tables, shells grouped by non-decreasing l as in the
+ * plan: grp_offsets/grp_l host arrays, ngroups groups).  No reference
+ * counterpart on this path: the reference's SCF driver takes them from
+ * PySCF/libcint on the CPU.  Returns 0 ok, 1 unsupported l / launch error. */
+int BQCint1e(double *S, double *T, double *V, const int *atm, int natm, const int *bas,
+             const double *env, const int *ao_loc, int nao, const int *grp_offsets, int ngroups,
+             const int *grp_l, hipStream_t stream);
+
 /* J = 2 (V + V^T), K = V + V^T in place on device n_dm x nao x nao arrays
  * (either may be NULL). */
 int BQCsymmetrize_jk(double *vj, double *vk, int nao, int n_dm, hipStream_t stream);

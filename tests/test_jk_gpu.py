@@ -134,3 +134,32 @@ def test_mixed_precision_c1_energy_bar():
     assert fl[1] > 0
     assert abs(e - eref) <= 1e-8
     assert dj <= 1e-8 and dk <= 1e-8
+
+
+@pytest.mark.parametrize('basis', ['cc-pvdz', 'def2-svp', 'cc-pvtz'])
+def test_int1e_matches_oracle(basis):
+    _torch_gpu()
+    from byteqc_amd.gto import M
+    from byteqc_amd.scf import hf, rhf
+    from oracle import scf_oracle as so
+    mol = M(atom='O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587; O 2.1 0.3 -0.4', basis=basis)
+    opt = hf._VHFOpt(mol).build()
+    got = [m.cpu().numpy() for m in rhf.int1e(opt)]
+    ref = so.int1e(mol)
+    for name, g, r in zip('STV', got, ref):
+        d = numpy.abs(g - r).max()
+        print(f'int1e {basis} {name}: max diff {d:.2e}', flush=True)
+        assert d < 1e-11
+
+
+def test_rhf_known_answer_and_mixed_schedule():
+    _torch_gpu()
+    from byteqc_amd.gto import M
+    from byteqc_amd.scf import rhf
+    mol = M(atom='O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587', basis='cc-pvdz')
+    e64, mf = rhf.scf_energy(mol, conv_tol=1e-10)
+    print(f'RHF fp64 {e64:.10f} in {mf.cycles} cycles', flush=True)
+    assert mf.converged and abs(e64 - (-76.0267656731)) < 1e-9
+    emx, mf2 = rhf.scf_energy(mol, conv_tol=1e-10, precision='mixed_then_fp64')
+    print(f'RHF mixed->fp64 {emx:.10f} builds {mf2.jk_builds}', flush=True)
+    assert mf2.converged and abs(emx - e64) < 1e-8

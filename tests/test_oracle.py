@@ -206,3 +206,24 @@ def _cart_overlap(atm, bas, env):
                         v *= one_d(ci_[d], cj_[d], ai, aj, ri[d], rj[d])
                     s[ao_loc[I] + x, ao_loc[J] + y] = v
     return s
+
+
+def test_oracle_rhf_known_answer():
+    """RHF/cc-pVDZ of water at O 0 0 0; H 0 -+0.757 0.587 (Angstrom):
+    -76.0267656731 Eh is the energy PySCF's examples print for this input; it
+    pins the transcribed basis, the normalisation and Cartesian-to-spherical
+    conventions, the one-electron integrals and the J/K path together."""
+    from oracle import scf_oracle as so
+    mol = M(atom='O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587', basis='cc-pvdz')
+    e, _ = so.rhf(mol)
+    assert abs(e - (-76.0267656731)) < 2e-10
+
+
+def test_oracle_int1e_normalised_and_symmetric():
+    from oracle import scf_oracle as so
+    mol = M(atom='O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587', basis='def2-svp')
+    s, t, v = so.int1e(mol)
+    assert numpy.allclose(numpy.diag(s), 1, atol=1e-13)
+    for m in (s, t, v):
+        assert numpy.allclose(m, m.T, atol=1e-14)
+    assert numpy.all(numpy.linalg.eigvalsh(t) > 0)
