@@ -11,16 +11,17 @@
 #include <algorithm>
 #include <vector>
 
-#include "jk_v2.h"
+#include "jk_wave.h"
 
 static int pair_id(int a, int b) { return a * (a + 1) / 2 + b; }
 
 static JKKernelTable g_kernels;
 static RowKernelTable g_rows;
+static WaveKernelTable g_waves;
 static bool g_registered = false;
 
 #define BQC_DECL_BRA(LA, LB) \
-    void bqc_register_bra_##LA##LB(JKKernelTable &t1, RowKernelTable &t2);
+    void bqc_register_bra_##LA##LB(JKKernelTable &t1, RowKernelTable &t2, WaveKernelTable &t3);
 BQC_DECL_BRA(0, 0)
 BQC_DECL_BRA(1, 0)
 BQC_DECL_BRA(1, 1)
@@ -34,16 +35,16 @@ BQC_DECL_BRA(3, 3)
 
 static void register_kernels() {
     if (g_registered) return;
-    bqc_register_bra_00(g_kernels, g_rows);
-    bqc_register_bra_10(g_kernels, g_rows);
-    bqc_register_bra_11(g_kernels, g_rows);
-    bqc_register_bra_20(g_kernels, g_rows);
-    bqc_register_bra_21(g_kernels, g_rows);
-    bqc_register_bra_22(g_kernels, g_rows);
-    bqc_register_bra_30(g_kernels, g_rows);
-    bqc_register_bra_31(g_kernels, g_rows);
-    bqc_register_bra_32(g_kernels, g_rows);
-    bqc_register_bra_33(g_kernels, g_rows);
+    bqc_register_bra_00(g_kernels, g_rows, g_waves);
+    bqc_register_bra_10(g_kernels, g_rows, g_waves);
+    bqc_register_bra_11(g_kernels, g_rows, g_waves);
+    bqc_register_bra_20(g_kernels, g_rows, g_waves);
+    bqc_register_bra_21(g_kernels, g_rows, g_waves);
+    bqc_register_bra_22(g_kernels, g_rows, g_waves);
+    bqc_register_bra_30(g_kernels, g_rows, g_waves);
+    bqc_register_bra_31(g_kernels, g_rows, g_waves);
+    bqc_register_bra_32(g_kernels, g_rows, g_waves);
+    bqc_register_bra_33(g_kernels, g_rows, g_waves);
     g_registered = true;
 }
 
@@ -53,7 +54,7 @@ static double g_last_flops[2] = {0, 0};
 // on the build's stream.  Records (class code, algorithmic FLOPs, events).
 struct LaunchRec {
     int cls;       // li*1000 + lj*100 + lk*10 + ll of the evaluated class
-    int kind;      // 0 quartet kernel, 1 row kernel; +2 for FP32
+    int kind;      // 0 quartet kernel, 1 row kernel, 4 wave kernel; +2 for FP32
     double flops;  // algorithmic FLOPs of the launch
     double nquart;
     hipEvent_t e0, e1;
@@ -277,6 +278,7 @@ struct RowLaunch {
     V2Task task;
     size_t off_perm_p, off_perm_q, off_items;
     int cls, prec;
+    int wave;  // 0: 64 P pairs per wavefront (row kernel), 1: one P pair per workgroup
     double flops, nquart;
 };
 
@@ -434,6 +436,106 @@ static int plan_rows(const TaskCtx &c, const int *bra_order, std::vector<int> &s
     return 0;
 }
 
+#define BQC_WAVE_CK 32
+
+// Plans the wavefront-per-quartet launches of one task: one item per P pair
+// and range of at most BQC_WAVE_CK Q pairs.  Returns -1 if the class has no
+// such kernel.
+static int plan_wave(const TaskCtx &c, const int *bra_order, std::vector<int> &stage,
+                     std::vector<RowLaunch> &rows, double *flops) {
+    WaveKernel k64 = c.swap ? g_waves[0][c.pkl][c.pij] : g_waves[0][c.pij][c.pkl];
+    WaveKernel k32 = c.swap ? g_waves[1][c.pkl][c.pij] : g_waves[1][c.pij][c.pkl];
+    if (!k64 || !k32 || !bra_order) return -1;
+    if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32)) return 1;
+    BasisProdCache *bp = c.bp;
+    const int nbins = c.nbins;
+    const int *pbins = c.swap ? c.bins_kl : c.bins_ij;
+    const int *qbins = c.swap ? c.bins_ij : c.bins_kl;
+    int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
+    const int *locs = bp->bas_pairs_locs;
+    int np = pbins[nbins], nq = qbins[nbins];
+    double nquart[2] = {0, 0};
+    for (int bq = 0; bq < nbins; ++bq) {
+        int q0 = qbins[bq], q1 = qbins[bq + 1];
+        if (q1 <= q0) continue;
+        double ubq = bin_ub(c, cp_q, qbins, bq);
+        for (int bpn = 0; bpn < nbins - bq; ++bpn) {
+            int p0 = pbins[bpn], p1 = pbins[bpn + 1];
+            if (p1 <= p0) continue;
+            int prec = bin_ub(c, cp_p, pbins, bpn) + ubq < c.log_cut ? 1 : 0;
+            nquart[prec] += count_rect(p0, p1, q0, q1, c.diag);
+        }
+    }
+    flops[0] += nquart[0] * c.qflops;
+    flops[1] += nquart[1] * c.qflops;
+    if (np == 0 || nq == 0) return 0;
+    RowLaunch r[2];
+    for (int prec = 0; prec < 2; ++prec) {
+        memset(&r[prec], 0, sizeof(RowLaunch));
+        r[prec].kern = prec ? k32 : k64;
+        r[prec].prec = prec;
+        r[prec].wave = 1;
+        r[prec].cls = eval_cls(c);
+        r[prec].flops = nquart[prec] * c.qflops;
+        r[prec].nquart = nquart[prec];
+        V2Task &tk = r[prec].task;
+        tk.np = np;
+        tk.bas_p0 = locs[cp_p];
+        tk.bas_q0 = locs[cp_q];
+        tk.prim_p0 = bp->primitive_pairs_locs[cp_p];
+        tk.prim_q0 = bp->primitive_pairs_locs[cp_q];
+        tk.nprim_p = bp->cptype[cp_p].nprim_12;
+        tk.nprim_q = bp->cptype[cp_q].nprim_12;
+        tk.diag = c.diag;
+        tk.nbins = nbins;
+        for (int b = 0; b <= nbins; ++b) {
+            tk.pbins[b] = pbins[b];
+            tk.qbins[b] = qbins[b];
+        }
+    }
+    size_t off_p = stage.size();
+    side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], stage);
+    size_t off_q = off_p;
+    if (cp_p != cp_q) {
+        off_q = stage.size();
+        side_order(qbins, nbins, bra_order + locs[cp_q], locs[cp_q + 1] - locs[cp_q], stage);
+    }
+    std::vector<int> items[2];
+    for (int p = 0; p < np; ++p) {
+        int s = (int)(std::upper_bound(pbins + 1, pbins + nbins + 1, p) - (pbins + 1));
+        int qmax = qbins[nbins - s];
+        double ubp = bin_ub(c, cp_p, pbins, s);
+        int split = nbins - s;
+        for (int bq = 0; bq < nbins - s; ++bq)
+            if (ubp + bin_ub(c, cp_q, qbins, bq) < c.log_cut) {
+                split = bq;
+                break;
+            }
+        int qsplit = qbins[split];
+        int lim[2][2] = {{0, qsplit < qmax ? qsplit : qmax}, {qsplit, qmax}};
+        for (int prec = 0; prec < 2; ++prec)
+            for (int q0 = lim[prec][0]; q0 < lim[prec][1]; q0 += BQC_WAVE_CK) {
+                int q1 = q0 + BQC_WAVE_CK < lim[prec][1] ? q0 + BQC_WAVE_CK : lim[prec][1];
+                items[prec].push_back(p);
+                items[prec].push_back(q0);
+                items[prec].push_back(q1);
+                items[prec].push_back(0);
+            }
+    }
+    for (int prec = 0; prec < 2; ++prec) {
+        int nitems = (int)items[prec].size() / 4;
+        if (!nitems) continue;
+        while (stage.size() & 3) stage.push_back(0);
+        r[prec].off_perm_p = off_p;
+        r[prec].off_perm_q = off_q;
+        r[prec].off_items = stage.size();
+        stage.insert(stage.end(), items[prec].begin(), items[prec].end());
+        r[prec].task.nitems = nitems;
+        rows.push_back(r[prec]);
+    }
+    return 0;
+}
+
 extern "C" {
 
 int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
@@ -470,6 +572,7 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
                             task_dmax ? task_dmax[t] : 0, fp32_cut, precision);
         if (err) return err;
         err = plan_rows(c, bra_order, stage, rows, flops);
+        if (err < 0) err = plan_wave(c, bra_order, stage, rows, flops);
         if (err < 0) err = launch_quartets(c, jk, pd, stream, flops);
         if (err > 0) return err;
     }
@@ -480,10 +583,12 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
             r.task.perm_p = g_dws + r.off_perm_p;
             r.task.perm_q = g_dws + r.off_perm_q;
             r.task.items = (const int4 *)(g_dws + r.off_items);
-            unsigned nblk = (unsigned)((r.task.nitems + 3) / 4);
+            unsigned nblk = r.wave ? (unsigned)r.task.nitems : (unsigned)((r.task.nitems + 3) / 4);
+            unsigned nthr = r.wave ? 64 : 256;
             LaunchRec rec;
-            if (g_prof) record_launch(rec, r.cls, 1 + 2 * r.prec, r.flops, r.nquart, stream);
-            hipLaunchKernelGGL(r.kern, dim3(nblk), dim3(256), 0, stream, jk, pd, r.task);
+            if (g_prof)
+                record_launch(rec, r.cls, (r.wave ? 4 : 1) + 2 * r.prec, r.flops, r.nquart, stream);
+            hipLaunchKernelGGL(r.kern, dim3(nblk), dim3(nthr), 0, stream, jk, pd, r.task);
             if (g_prof) {
                 hipEventRecord(rec.e1, stream);
                 g_recs.push_back(rec);

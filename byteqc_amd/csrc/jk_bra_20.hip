@@ -1,5 +1,7 @@
 // Kernel instantiations of the J/K build for bra class (20| and all
 // smaller ket classes.
-#include "jk_v2.h"
+#include "jk_wave.h"
 
-void bqc_register_bra_20(JKKernelTable &t1, RowKernelTable &t2) { reg_bra<2, 0>(t1, t2); }
+void bqc_register_bra_20(JKKernelTable &t1, RowKernelTable &t2, WaveKernelTable &t3) {
+    reg_bra<2, 0>(t1, t2, t3);
+}

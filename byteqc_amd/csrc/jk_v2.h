@@ -280,24 +280,3 @@ constexpr bool use_rows() {
     return Cls<LA, LB, LC, LD>::NF <= 81;
 }
 
-template <int LA, int LB, int LC, int LD>
-static void reg_class(JKKernelTable &t1, RowKernelTable &t2) {
-    reg_kernel<LA, LB, LC, LD>(t1);
-    if constexpr (use_rows<LA, LB, LC, LD>()) reg_row<LA, LB, LC, LD>(t2);
-}
-
-// all ket classes (LC LD) with pair-id(LC,LD) <= pair-id(LA,LB)
-template <int LA, int LB>
-static void reg_bra(JKKernelTable &t1, RowKernelTable &t2) {
-    constexpr int pab = LA * (LA + 1) / 2 + LB;
-    reg_class<LA, LB, 0, 0>(t1, t2);
-    if constexpr (pab >= 1) reg_class<LA, LB, 1, 0>(t1, t2);
-    if constexpr (pab >= 2) reg_class<LA, LB, 1, 1>(t1, t2);
-    if constexpr (pab >= 3) reg_class<LA, LB, 2, 0>(t1, t2);
-    if constexpr (pab >= 4) reg_class<LA, LB, 2, 1>(t1, t2);
-    if constexpr (pab >= 5) reg_class<LA, LB, 2, 2>(t1, t2);
-    if constexpr (pab >= 6) reg_class<LA, LB, 3, 0>(t1, t2);
-    if constexpr (pab >= 7) reg_class<LA, LB, 3, 1>(t1, t2);
-    if constexpr (pab >= 8) reg_class<LA, LB, 3, 2>(t1, t2);
-    if constexpr (pab >= 9) reg_class<LA, LB, 3, 3>(t1, t2);
-}

@@ -1,0 +1,320 @@
+// Wavefront-per-quartet J/K kernel for classes with large integral blocks.
+//
+// A 64-lane workgroup owns one P pair (a b) and walks a range of the Q list
+// (c d) -- the same bin-major, first-shell-minor lists and work items as the
+// row kernel, with one P pair per item instead of 64.  For every quartet:
+//   A. lanes t < 3*NR evaluate the 2-D Rys table of direction t%3, root t/3
+//      (FP64 roots, T-typed recurrence) into LDS;
+//   B. the 64 lanes split the Cartesian components and accumulate the
+//      contracted block over roots and primitive quartets in registers;
+//   C. the block goes to LDS and the lanes split the elements of the six
+//      digestion targets:
+//        J(a,b) += g D(c,d)  lane registers, flushed once per item
+//        K(a,c) += g D(b,d)  lane registers over a run of equal c
+//        K(b,c) += g D(a,d)  lane registers over a run of equal c
+//        J(c,d), K(a,d), K(b,d): one FP64 atomic per element and quartet.
+// LDS per workgroup: 3*NR*GSZ + NF values of type T plus the density blocks.
+#pragma once
+#include "jk_v2.h"
+
+template <int LA, int LB, int LC, int LD>
+struct WaveCls {
+    using C = Cls<LA, LB, LC, LD>;
+    static constexpr int NA = C::NFI, NB = C::NFJ, NC = C::NFK, ND = C::NFL;
+    static constexpr int NG = (C::NF + 63) / 64;              // block values per lane
+    static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NAD = NA * ND,
+                         NBC = NB * NC, NBD = NB * ND;
+    static constexpr int EAB = (NAB + 63) / 64, EAC = (NAC + 63) / 64, EBC = (NBC + 63) / 64;
+};
+
+// I-table index of component n = ((x*NB + y)*NC + z)*ND + w for direction d
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ int gidx(int n, int d) {
+    using W = WaveCls<LA, LB, LC, LD>;
+    using C = typename W::C;
+    const int w = n % W::ND, z = (n / W::ND) % W::NC, y = (n / (W::ND * W::NC)) % W::NB,
+              x = n / (W::ND * W::NC * W::NB);
+    auto e = [d](int l, int c) { return d == 0 ? cart_x(l, c) : (d == 1 ? cart_y(l, c) : cart_z(l, c)); };
+    return C::g(e(LA, x), e(LB, y), e(LC, z), e(LD, w));
+}
+
+template <int LA, int LB, int LC, int LD, typename T>
+__global__ void __launch_bounds__(64) jk_wave(JKArgs jk, PairData pd, V2Task tk) {
+    using W = WaveCls<LA, LB, LC, LD>;
+    using C = typename W::C;
+    constexpr int NR = C::NR, NA = W::NA, NB = W::NB, NC = W::NC, ND = W::ND;
+    __shared__ T sI[3 * NR * C::GSZ];
+    __shared__ T sg[C::NF];
+    __shared__ double sD[W::NAB + W::NCD + W::NBD + W::NAD + W::NBC + W::NAC];
+    double *sDab = sD, *sDcd = sDab + W::NAB, *sDbd = sDcd + W::NCD, *sDad = sDbd + W::NBD,
+           *sDbc = sDad + W::NAD, *sDac = sDbc + W::NBC;
+
+    const int lane = threadIdx.x;
+    const int item = blockIdx.x;
+    if (item >= tk.nitems) return;
+    const int4 it = tk.items[item];
+    const int pidx = it.x;
+    int s = 0;
+    while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
+    const int q0 = it.y;
+    const int q1 = min(it.z, tk.qbins[tk.nbins - s]);
+    if (q1 <= q0) return;
+    const int ppos = tk.perm_p[pidx];
+    const int gp = tk.bas_p0 + ppos;
+    const int sa = pd.bra[gp], sb = pd.ket[gp];
+    const int pp0 = tk.prim_p0 + ppos * tk.nprim_p;
+    const int a0 = pd.ao_loc[sa], b0 = pd.ao_loc[sb];
+    const double ra[3] = {pd.bas_x[sa], pd.bas_y[sa], pd.bas_z[sa]};
+    const double rb[3] = {pd.bas_x[sb], pd.bas_y[sb], pd.bas_z[sb]};
+    const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+    // I-table indices of this lane's components
+    int gi[W::NG][3];
+#pragma unroll
+    for (int m = 0; m < W::NG; ++m) {
+        const int n = lane + 64 * m;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gi[m][d] = n < C::NF ? gidx<LA, LB, LC, LD>(n, d) : 0;
+    }
+    const int nao = jk.nao;
+    const size_t nao2 = (size_t)nao * nao;
+    for (int idm = 0; idm < jk.n_dm; ++idm) {
+        const double *__restrict__ D = jk.dm + idm * nao2;
+        double *VJ = jk.vj ? jk.vj + idm * nao2 : nullptr;
+        double *VK = jk.vk ? jk.vk + idm * nao2 : nullptr;
+        for (int e = lane; e < W::NAB; e += 64)
+            sDab[e] = D[(size_t)(a0 + e / NB) * nao + b0 + e % NB];
+        double jab[W::EAB], kac[W::EAC], kbc[W::EBC];
+#pragma unroll
+        for (int m = 0; m < W::EAB; ++m) jab[m] = 0.0;
+#pragma unroll
+        for (int m = 0; m < W::EAC; ++m) kac[m] = 0.0;
+#pragma unroll
+        for (int m = 0; m < W::EBC; ++m) kbc[m] = 0.0;
+        int sc = pd.bra[tk.bas_q0 + tk.perm_q[q0]];
+        for (int q = q0; q < q1; ++q) {
+            const int qpos = tk.perm_q[q];
+            const int gq = tk.bas_q0 + qpos;
+            const int sd = pd.ket[gq];
+            const int c0 = pd.ao_loc[sc], d0 = pd.ao_loc[sd];
+            const bool skip = tk.diag && ppos < qpos;
+            double norm = TWO_PI_52;
+            if (tk.diag && ppos == qpos) norm *= 0.5;
+            T g[W::NG];
+#pragma unroll
+            for (int m = 0; m < W::NG; ++m) g[m] = (T)0;
+            if (!skip) {
+                const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
+                const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
+                const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
+                const int qq0 = tk.prim_q0 + qpos * tk.nprim_q;
+                for (int ip = 0; ip < tk.nprim_p; ++ip) {
+                    const int p = pp0 + ip;
+                    const double aij = pd.a12[p], eij = pd.e12[p];
+                    const double P[3] = {pd.x12[p], pd.y12[p], pd.z12[p]};
+                    for (int iq = 0; iq < tk.nprim_q; ++iq) {
+                        const int qq = qq0 + iq;
+                        const double akl = pd.a12[qq], ekl = pd.e12[qq];
+                        const double Q[3] = {pd.x12[qq], pd.y12[qq], pd.z12[qq]};
+                        const double pq[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+                        const double aijkl = aij + akl, a1 = aij * akl, a0q = a1 / aijkl;
+                        const double x = a0q * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]);
+                        // phase A: one (direction, root) table per lane
+                        if (lane < 3 * NR) {
+                            const int d = lane % 3, r = lane / 3;
+                            double u[NR], w[NR];
+                            rys_roots<NR>(x, u, w);
+                            double ur = u[0], wr = w[0];
+#pragma unroll
+                            for (int k = 1; k < NR; ++k)
+                                if (r == k) {
+                                    ur = u[k];
+                                    wr = w[k];
+                                }
+                            const double u2 = a0q * ur;
+                            const double tmp4 = 0.5 / (u2 * aijkl + a1);
+                            const double b00 = u2 * tmp4;
+                            const double b10 = b00 + tmp4 * akl;
+                            const double b01 = b00 + tmp4 * aij;
+                            const double seed = d == 0 ? norm
+                                                : (d == 1 ? eij * ekl / (sqrt(aijkl) * a1) : wr);
+                            const double Pd = d == 0 ? P[0] : (d == 1 ? P[1] : P[2]);
+                            const double Qd = d == 0 ? Q[0] : (d == 1 ? Q[1] : Q[2]);
+                            const double Ad = d == 0 ? ra[0] : (d == 1 ? ra[1] : ra[2]);
+                            const double Cd = d == 0 ? rc[0] : (d == 1 ? rc[1] : rc[2]);
+                            const double pqd = Pd - Qd;
+                            const double abd = d == 0 ? ab[0] : (d == 1 ? ab[1] : ab[2]);
+                            const double cdd = d == 0 ? cd[0] : (d == 1 ? cd[1] : cd[2]);
+                            rys2d<LA, LB, LC, LD, T>(sI + (r * 3 + d) * C::GSZ, (T)seed,
+                                                     (T)((Pd - Ad) - 2.0 * b00 * akl * pqd),
+                                                     (T)((Qd - Cd) + 2.0 * b00 * aij * pqd), (T)b10,
+                                                     (T)b01, (T)b00, (T)abd, (T)cdd);
+                        }
+                        __syncthreads();
+                        // phase B: components
+#pragma unroll
+                        for (int m = 0; m < W::NG; ++m) {
+                            if (lane + 64 * m < C::NF) {
+                                T acc = (T)0;
+#pragma unroll
+                                for (int r = 0; r < NR; ++r) {
+                                    const T *I0 = sI + (r * 3) * C::GSZ;
+                                    acc += I0[gi[m][0]] * I0[C::GSZ + gi[m][1]] *
+                                           I0[2 * C::GSZ + gi[m][2]];
+                                }
+                                g[m] += acc;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+            // phase C: digestion
+#pragma unroll
+            for (int m = 0; m < W::NG; ++m)
+                if (lane + 64 * m < C::NF) sg[lane + 64 * m] = g[m];
+            for (int e = lane; e < W::NCD; e += 64)
+                sDcd[e] = D[(size_t)(c0 + e / ND) * nao + d0 + e % ND];
+            for (int e = lane; e < W::NBD; e += 64)
+                sDbd[e] = D[(size_t)(d0 + e % ND) * nao + b0 + e / ND];
+            for (int e = lane; e < W::NAD; e += 64)
+                sDad[e] = D[(size_t)(d0 + e % ND) * nao + a0 + e / ND];
+            for (int e = lane; e < W::NBC; e += 64)
+                sDbc[e] = D[(size_t)(c0 + e % NC) * nao + b0 + e / NC];
+            for (int e = lane; e < W::NAC; e += 64)
+                sDac[e] = D[(size_t)(c0 + e % NC) * nao + a0 + e / NC];
+            __syncthreads();
+#define SG(x, y, z, w) ((double)sg[(((x) * NB + (y)) * NC + (z)) * ND + (w)])
+            if (!skip) {
+                if (VJ) {
+#pragma unroll
+                    for (int m = 0; m < W::EAB; ++m) {
+                        const int e = lane + 64 * m;
+                        if (e < W::NAB) {
+                            const int x = e / NB, y = e % NB;
+                            double sum = 0.0;
+                            for (int z = 0; z < NC; ++z)
+                                for (int w = 0; w < ND; ++w) sum += SG(x, y, z, w) * sDcd[z * ND + w];
+                            jab[m] += sum;
+                        }
+                    }
+                    for (int e = lane; e < W::NCD; e += 64) {
+                        const int z = e / ND, w = e % ND;
+                        double sum = 0.0;
+                        for (int x = 0; x < NA; ++x)
+                            for (int y = 0; y < NB; ++y) sum += SG(x, y, z, w) * sDab[x * NB + y];
+                        atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, sum);
+                    }
+                }
+                if (VK) {
+#pragma unroll
+                    for (int m = 0; m < W::EAC; ++m) {
+                        const int e = lane + 64 * m;
+                        if (e < W::NAC) {
+                            const int x = e / NC, z = e % NC;
+                            double sum = 0.0;
+                            for (int y = 0; y < NB; ++y)
+                                for (int w = 0; w < ND; ++w) sum += SG(x, y, z, w) * sDbd[y * ND + w];
+                            kac[m] += sum;
+                        }
+                    }
+#pragma unroll
+                    for (int m = 0; m < W::EBC; ++m) {
+                        const int e = lane + 64 * m;
+                        if (e < W::NBC) {
+                            const int y = e / NC, z = e % NC;
+                            double sum = 0.0;
+                            for (int x = 0; x < NA; ++x)
+                                for (int w = 0; w < ND; ++w) sum += SG(x, y, z, w) * sDad[x * ND + w];
+                            kbc[m] += sum;
+                        }
+                    }
+                    for (int e = lane; e < W::NAD; e += 64) {
+                        const int x = e / ND, w = e % ND;
+                        double sum = 0.0;
+                        for (int y = 0; y < NB; ++y)
+                            for (int z = 0; z < NC; ++z) sum += SG(x, y, z, w) * sDbc[y * NC + z];
+                        atomic_add(VK + (size_t)(d0 + w) * nao + a0 + x, sum);
+                    }
+                    for (int e = lane; e < W::NBD; e += 64) {
+                        const int y = e / ND, w = e % ND;
+                        double sum = 0.0;
+                        for (int x = 0; x < NA; ++x)
+                            for (int z = 0; z < NC; ++z) sum += SG(x, y, z, w) * sDac[x * NC + z];
+                        atomic_add(VK + (size_t)(d0 + w) * nao + b0 + y, sum);
+                    }
+                }
+            }
+#undef SG
+            __syncthreads();
+            // end of a run of equal c: flush K(a,c), K(b,c)
+            const int sc_next = q + 1 < q1 ? pd.bra[tk.bas_q0 + tk.perm_q[q + 1]] : -1;
+            if (VK && sc_next != sc) {
+#pragma unroll
+                for (int m = 0; m < W::EAC; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NAC && kac[m] != 0.0)
+                        atomic_add(VK + (size_t)(c0 + e % NC) * nao + a0 + e / NC, kac[m]);
+                    kac[m] = 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < W::EBC; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NBC && kbc[m] != 0.0)
+                        atomic_add(VK + (size_t)(c0 + e % NC) * nao + b0 + e / NC, kbc[m]);
+                    kbc[m] = 0.0;
+                }
+            }
+            sc = sc_next;
+        }
+        if (VJ) {
+#pragma unroll
+            for (int m = 0; m < W::EAB; ++m) {
+                const int e = lane + 64 * m;
+                if (e < W::NAB && jab[m] != 0.0)
+                    atomic_add(VJ + (size_t)(a0 + e / NB) * nao + b0 + e % NB, jab[m]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+typedef RowKernel WaveKernel;
+typedef WaveKernel WaveKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
+
+// Classes digested with the wavefront-per-quartet kernel (LDS budget).
+template <int LA, int LB, int LC, int LD>
+constexpr bool use_wave() {
+    using C = Cls<LA, LB, LC, LD>;
+    return C::NF > 81 && (3 * C::NR * C::GSZ + C::NF) * 8 <= 40000;
+}
+
+template <int LA, int LB, int LC, int LD>
+static void reg_wave(WaveKernelTable &tab) {
+    if constexpr (use_wave<LA, LB, LC, LD>()) {
+        tab[0][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_wave<LA, LB, LC, LD, double>;
+        tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_wave<LA, LB, LC, LD, float>;
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+static void reg_class(JKKernelTable &t1, RowKernelTable &t2, WaveKernelTable &t3) {
+    reg_kernel<LA, LB, LC, LD>(t1);
+    if constexpr (use_rows<LA, LB, LC, LD>()) reg_row<LA, LB, LC, LD>(t2);
+    reg_wave<LA, LB, LC, LD>(t3);
+}
+
+// all ket classes (LC LD) with pair-id(LC,LD) <= pair-id(LA,LB)
+template <int LA, int LB>
+static void reg_bra(JKKernelTable &t1, RowKernelTable &t2, WaveKernelTable &t3) {
+    constexpr int pab = LA * (LA + 1) / 2 + LB;
+    reg_class<LA, LB, 0, 0>(t1, t2, t3);
+    if constexpr (pab >= 1) reg_class<LA, LB, 1, 0>(t1, t2, t3);
+    if constexpr (pab >= 2) reg_class<LA, LB, 1, 1>(t1, t2, t3);
+    if constexpr (pab >= 3) reg_class<LA, LB, 2, 0>(t1, t2, t3);
+    if constexpr (pab >= 4) reg_class<LA, LB, 2, 1>(t1, t2, t3);
+    if constexpr (pab >= 5) reg_class<LA, LB, 2, 2>(t1, t2, t3);
+    if constexpr (pab >= 6) reg_class<LA, LB, 3, 0>(t1, t2, t3);
+    if constexpr (pab >= 7) reg_class<LA, LB, 3, 1>(t1, t2, t3);
+    if constexpr (pab >= 8) reg_class<LA, LB, 3, 2>(t1, t2, t3);
+    if constexpr (pab >= 9) reg_class<LA, LB, 3, 3>(t1, t2, t3);
+}
