@@ -1,7 +1,9 @@
 """Build libbqcjk.so (HIP kernels for gfx950 + host C++) in-tree.
 
-    python -m byteqc_amd.build
+    python -m byteqc_amd.build [--incremental]
 
+--incremental recompiles only sources older than their object or than the
+newest header under csrc/ and include/.
 Objects go to byteqc_amd/_build/, the library to byteqc_amd/lib/libbqcjk.so.
 """
 import concurrent.futures
@@ -18,7 +20,8 @@ ARCH = os.environ.get('BQC_OFFLOAD_ARCH', 'gfx950')
 
 SOURCES = ['jk.hip', 'jk_bra_00.hip', 'jk_bra_10.hip', 'jk_bra_11.hip', 'jk_bra_20.hip',
            'jk_bra_21.hip', 'jk_bra_22.hip', 'jk_bra_30.hip', 'jk_bra_31.hip', 'jk_bra_32.hip',
-           'jk_bra_33.hip', 'schwarz.hip', 'int1e.hip', 'pair_data.cpp']
+           'jk_bra_33.hip', 'schwarz.hip', 'int1e.hip', 'dmcond.hip', 'pair_data.cpp',
+           'planner.cpp']
 HIP_FLAGS = ['-O3', '-fPIC', '-std=c++17', f'--offload-arch={ARCH}', '-munsafe-fp-atomics',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-Wno-unused-command-line-argument']
@@ -31,8 +34,19 @@ def _hipcc():
     raise RuntimeError('hipcc not found')
 
 
-def _compile(src):
+def _newest_header():
+    inc = os.path.join(os.path.dirname(HERE), 'include')
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(('.h', '.inc'))]
+    paths += [os.path.join(inc, f) for f in os.listdir(inc)]
+    return max(os.path.getmtime(p) for p in paths)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + '.o')
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(os.path.join(CSRC, src)) and t > _newest_header():
+            return obj
     cmd = [_hipcc(), *HIP_FLAGS, '-c', os.path.join(CSRC, src), '-o', obj]
     if src.endswith('.cpp'):
         cmd = [_hipcc(), '-O3', '-fPIC', '-std=c++17', '-c', os.path.join(CSRC, src), '-o', obj]
@@ -42,11 +56,11 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB + '.tmp'
     cmd = [_hipcc(), '-shared', f'--offload-arch={ARCH}', '-o', tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,5 +73,5 @@ def build(verbose=False):
 
 
 if __name__ == '__main__':
-    build(verbose=True)
+    build(verbose=True, incremental='--incremental' in sys.argv[1:])
     sys.exit(0)

@@ -271,78 +271,74 @@ static PairData make_pair_data(const BasisProdCache *bp) {
 }
 
 // ---------------------------------------------------------------------------
-// wavefront-stationary kernels: host-side list order and work items
+// Row and wave kernels: host-side list orders, task descriptors and items.
+// A BuildPlan collects the tasks of one or many molecules; items are grouped
+// by kernel so that every (class, precision, kernel kind) is one launch.
 // ---------------------------------------------------------------------------
-struct RowLaunch {
+#define BQC_WAVE_CK 32
+
+static int g_ck_rows = 64;   // Q pairs per row-kernel item
+static int g_ck_wave = 32;   // Q pairs per wave-kernel item
+static std::vector<std::pair<long long, int>> g_ck_override;  // (key, ck)
+
+static int tile_for(int kind, int cls, int prec) {
+    long long key = ((long long)kind * 10000 + cls) * 2 + prec;
+    for (auto &kv : g_ck_override)
+        if (kv.first == key) return kv.second;
+    return kind == 1 ? g_ck_rows : g_ck_wave;
+}
+
+struct Group {
     RowKernel kern;
-    V2Task task;
-    size_t off_perm_p, off_perm_q, off_items;
-    int cls, prec;
-    int wave;  // 0: 64 P pairs per wavefront (row kernel), 1: one P pair per workgroup
+    int wave, cls, prec;
     double flops, nquart;
+    std::vector<int> items;  // int4 per item
 };
 
-static int *g_dws = nullptr;         // device workspace (ints)
-static size_t g_dws_cap = 0;
-static int *g_hstage = nullptr;      // pinned staging
-static size_t g_hstage_cap = 0;
-static hipEvent_t g_stage_evt = nullptr;
-static bool g_stage_pending = false;
+struct BuildPlan {
+    std::vector<TaskDesc> tasks;
+    std::vector<size_t> perm_p_off, perm_q_off;  // in ints
+    std::vector<int> ints;                       // list orders
+    std::vector<Group> groups;
+    double flops[2] = {0, 0};
 
-static int stage_to_device(const std::vector<int> &stage, hipStream_t stream) {
-    size_t n = stage.size();
-    if (g_stage_pending) {
-        hipEventSynchronize(g_stage_evt);
-        g_stage_pending = false;
+    Group &group(RowKernel k, int wave, int cls, int prec) {
+        for (Group &g : groups)
+            if (g.kern == k) return g;
+        groups.push_back(Group{k, wave, cls, prec, 0, 0, {}});
+        return groups.back();
     }
-    if (n > g_hstage_cap) {
-        if (g_hstage) hipHostFree(g_hstage);
-        size_t cap = n + n / 2 + 1024;
-        if (hipHostMalloc((void **)&g_hstage, cap * sizeof(int), hipHostMallocDefault) != hipSuccess)
-            return 1;
-        g_hstage_cap = cap;
-    }
-    if (n > g_dws_cap) {
-        // the previous workspace may still be read by queued kernels
-        hipStreamSynchronize(stream);
-        if (g_dws) hipFree(g_dws);
-        size_t cap = n + n / 2 + 1024;
-        if (hipMalloc((void **)&g_dws, cap * sizeof(int)) != hipSuccess) return 1;
-        g_dws_cap = cap;
-    }
-    if (!g_stage_evt) hipEventCreateWithFlags(&g_stage_evt, hipEventDisableTiming);
-    memcpy(g_hstage, stage.data(), n * sizeof(int));
-    if (hipMemcpyAsync(g_dws, g_hstage, n * sizeof(int), hipMemcpyHostToDevice, stream) !=
-        hipSuccess)
-        return 1;
-    hipEventRecord(g_stage_evt, stream);
-    g_stage_pending = true;
-    return 0;
-}
+};
 
 // List order of one side: plan positions [0, bins[nbins]) grouped by bin,
 // inside a bin by first shell (bra_order gives the plan positions of the
-// pair type sorted by first shell).
+// pair type sorted by first shell, then second shell).
 static void side_order(const int *bins, int nbins, const int *order, int npairs_cp,
-                       std::vector<int> &stage) {
+                       std::vector<int> &out) {
     const int n = bins[nbins];
     std::vector<int> cursor(bins, bins + nbins);
-    size_t base = stage.size();
-    stage.resize(base + n);
+    size_t base = out.size();
+    out.resize(base + n);
     for (int k = 0; k < npairs_cp; ++k) {
         int pos = order[k];
         if (pos >= n) continue;
         int b = (int)(std::upper_bound(bins + 1, bins + nbins + 1, pos) - (bins + 1));
-        stage[base + cursor[b]++] = pos;
+        out[base + cursor[b]++] = pos;
     }
 }
 
-// Plans the row-kernel launches of one task (FP64 and FP32 item lists).
-// Returns -1 if the class has no row kernel (caller falls back).
-static int plan_rows(const TaskCtx &c, const int *bra_order, std::vector<int> &stage,
-                     std::vector<RowLaunch> &rows, double *flops) {
-    RowKernel k64 = c.swap ? g_rows[0][c.pkl][c.pij] : g_rows[0][c.pij][c.pkl];
-    RowKernel k32 = c.swap ? g_rows[1][c.pkl][c.pij] : g_rows[1][c.pij][c.pkl];
+// Adds one task to the plan with row (wave == 0) or wave (wave == 1)
+// kernels.  Returns -1 if the class has no such kernel.
+static int plan_task(BuildPlan &bp_, const TaskCtx &c, int wave, const int *bra_order,
+                     const JKArgs &jk, const PairData &pd) {
+    RowKernel k64, k32;
+    if (wave) {
+        k64 = c.swap ? g_waves[0][c.pkl][c.pij] : g_waves[0][c.pij][c.pkl];
+        k32 = c.swap ? g_waves[1][c.pkl][c.pij] : g_waves[1][c.pij][c.pkl];
+    } else {
+        k64 = c.swap ? g_rows[0][c.pkl][c.pij] : g_rows[0][c.pij][c.pkl];
+        k32 = c.swap ? g_rows[1][c.pkl][c.pij] : g_rows[1][c.pij][c.pkl];
+    }
     if (!k64 || !k32 || !bra_order) return -1;
     if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32)) return 1;
     BasisProdCache *bp = c.bp;
@@ -352,7 +348,7 @@ static int plan_rows(const TaskCtx &c, const int *bra_order, std::vector<int> &s
     int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
     const int *locs = bp->bas_pairs_locs;
     int np = pbins[nbins], nq = qbins[nbins];
-    // FLOPs split at bin granularity (total is exact)
+    // FLOPs, split between precisions at bin granularity (total is exact)
     double nquart[2] = {0, 0};
     for (int bq = 0; bq < nbins; ++bq) {
         int q0 = qbins[bq], q1 = qbins[bq + 1];
@@ -365,143 +361,52 @@ static int plan_rows(const TaskCtx &c, const int *bra_order, std::vector<int> &s
             nquart[prec] += count_rect(p0, p1, q0, q1, c.diag);
         }
     }
-    flops[0] += nquart[0] * c.qflops;
-    flops[1] += nquart[1] * c.qflops;
+    bp_.flops[0] += nquart[0] * c.qflops;
+    bp_.flops[1] += nquart[1] * c.qflops;
     if (np == 0 || nq == 0) return 0;
-    RowLaunch r[2];
-    for (int prec = 0; prec < 2; ++prec) {
-        memset(&r[prec], 0, sizeof(RowLaunch));
-        r[prec].kern = prec ? k32 : k64;
-        r[prec].prec = prec;
-        r[prec].cls = eval_cls(c);
-        r[prec].flops = nquart[prec] * c.qflops;
-        r[prec].nquart = nquart[prec];
-        V2Task &tk = r[prec].task;
-        tk.np = np;
-        tk.bas_p0 = locs[cp_p];
-        tk.bas_q0 = locs[cp_q];
-        tk.prim_p0 = bp->primitive_pairs_locs[cp_p];
-        tk.prim_q0 = bp->primitive_pairs_locs[cp_q];
-        tk.nprim_p = bp->cptype[cp_p].nprim_12;
-        tk.nprim_q = bp->cptype[cp_q].nprim_12;
-        tk.diag = c.diag;
-        tk.nbins = nbins;
-        for (int b = 0; b <= nbins; ++b) {
-            tk.pbins[b] = pbins[b];
-            tk.qbins[b] = qbins[b];
-        }
+    const int ti = (int)bp_.tasks.size();
+    TaskDesc td;
+    memset(&td, 0, sizeof(td));
+    td.pd = pd;
+    td.dm = jk.dm;
+    td.vj = jk.vj;
+    td.vk = jk.vk;
+    td.nao = jk.nao;
+    td.n_dm = jk.n_dm;
+    td.np = np;
+    td.bas_p0 = locs[cp_p];
+    td.bas_q0 = locs[cp_q];
+    td.prim_p0 = bp->primitive_pairs_locs[cp_p];
+    td.prim_q0 = bp->primitive_pairs_locs[cp_q];
+    td.nprim_p = bp->cptype[cp_p].nprim_12;
+    td.nprim_q = bp->cptype[cp_q].nprim_12;
+    td.diag = c.diag;
+    td.nbins = nbins;
+    for (int b = 0; b <= nbins; ++b) {
+        td.pbins[b] = pbins[b];
+        td.qbins[b] = qbins[b];
     }
-    size_t off_p = stage.size();
-    side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], stage);
+    bp_.tasks.push_back(td);
+    size_t off_p = bp_.ints.size();
+    side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], bp_.ints);
     size_t off_q = off_p;
     if (cp_p != cp_q) {
-        off_q = stage.size();
-        side_order(qbins, nbins, bra_order + locs[cp_q], locs[cp_q + 1] - locs[cp_q], stage);
+        off_q = bp_.ints.size();
+        side_order(qbins, nbins, bra_order + locs[cp_q], locs[cp_q + 1] - locs[cp_q], bp_.ints);
     }
-    std::vector<int> items[2];
-    int nchunk = (np + 63) / 64;
-    for (int ch = 0; ch < nchunk; ++ch) {
-        int first = ch * 64;
-        int s = (int)(std::upper_bound(pbins + 1, pbins + nbins + 1, first) - (pbins + 1));
-        int qmax = qbins[nbins - s];
-        double ubp = bin_ub(c, cp_p, pbins, s);
-        int split = nbins - s;
-        for (int bq = 0; bq < nbins - s; ++bq)
-            if (ubp + bin_ub(c, cp_q, qbins, bq) < c.log_cut) {
-                split = bq;
-                break;
-            }
-        int qsplit = qbins[split];
-        int lim[2][2] = {{0, qsplit < qmax ? qsplit : qmax}, {qsplit, qmax}};
-        for (int prec = 0; prec < 2; ++prec)
-            for (int q0 = lim[prec][0]; q0 < lim[prec][1]; q0 += BQC_V2_CK) {
-                int q1 = q0 + BQC_V2_CK < lim[prec][1] ? q0 + BQC_V2_CK : lim[prec][1];
-                items[prec].push_back(ch);
-                items[prec].push_back(q0);
-                items[prec].push_back(q1);
-                items[prec].push_back(0);
-            }
-    }
+    bp_.perm_p_off.push_back(off_p);
+    bp_.perm_q_off.push_back(off_q);
+    const int cls = eval_cls(c);
+    Group *g[2] = {&bp_.group(k64, wave, cls, 0), nullptr};
+    g[1] = &bp_.group(k32, wave, cls, 1);
+    g[0] = &bp_.group(k64, wave, cls, 0);  // re-fetch: group() may reallocate
     for (int prec = 0; prec < 2; ++prec) {
-        int nitems = (int)items[prec].size() / 4;
-        if (!nitems) continue;
-        while (stage.size() & 3) stage.push_back(0);  // int4 alignment
-        r[prec].off_perm_p = off_p;
-        r[prec].off_perm_q = off_q;
-        r[prec].off_items = stage.size();
-        stage.insert(stage.end(), items[prec].begin(), items[prec].end());
-        r[prec].task.nitems = nitems;
-        rows.push_back(r[prec]);
+        g[prec]->flops += nquart[prec] * c.qflops;
+        g[prec]->nquart += nquart[prec];
     }
-    return 0;
-}
-
-#define BQC_WAVE_CK 32
-
-// Plans the wavefront-per-quartet launches of one task: one item per P pair
-// and range of at most BQC_WAVE_CK Q pairs.  Returns -1 if the class has no
-// such kernel.
-static int plan_wave(const TaskCtx &c, const int *bra_order, std::vector<int> &stage,
-                     std::vector<RowLaunch> &rows, double *flops) {
-    WaveKernel k64 = c.swap ? g_waves[0][c.pkl][c.pij] : g_waves[0][c.pij][c.pkl];
-    WaveKernel k32 = c.swap ? g_waves[1][c.pkl][c.pij] : g_waves[1][c.pij][c.pkl];
-    if (!k64 || !k32 || !bra_order) return -1;
-    if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32)) return 1;
-    BasisProdCache *bp = c.bp;
-    const int nbins = c.nbins;
-    const int *pbins = c.swap ? c.bins_kl : c.bins_ij;
-    const int *qbins = c.swap ? c.bins_ij : c.bins_kl;
-    int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
-    const int *locs = bp->bas_pairs_locs;
-    int np = pbins[nbins], nq = qbins[nbins];
-    double nquart[2] = {0, 0};
-    for (int bq = 0; bq < nbins; ++bq) {
-        int q0 = qbins[bq], q1 = qbins[bq + 1];
-        if (q1 <= q0) continue;
-        double ubq = bin_ub(c, cp_q, qbins, bq);
-        for (int bpn = 0; bpn < nbins - bq; ++bpn) {
-            int p0 = pbins[bpn], p1 = pbins[bpn + 1];
-            if (p1 <= p0) continue;
-            int prec = bin_ub(c, cp_p, pbins, bpn) + ubq < c.log_cut ? 1 : 0;
-            nquart[prec] += count_rect(p0, p1, q0, q1, c.diag);
-        }
-    }
-    flops[0] += nquart[0] * c.qflops;
-    flops[1] += nquart[1] * c.qflops;
-    if (np == 0 || nq == 0) return 0;
-    RowLaunch r[2];
-    for (int prec = 0; prec < 2; ++prec) {
-        memset(&r[prec], 0, sizeof(RowLaunch));
-        r[prec].kern = prec ? k32 : k64;
-        r[prec].prec = prec;
-        r[prec].wave = 1;
-        r[prec].cls = eval_cls(c);
-        r[prec].flops = nquart[prec] * c.qflops;
-        r[prec].nquart = nquart[prec];
-        V2Task &tk = r[prec].task;
-        tk.np = np;
-        tk.bas_p0 = locs[cp_p];
-        tk.bas_q0 = locs[cp_q];
-        tk.prim_p0 = bp->primitive_pairs_locs[cp_p];
-        tk.prim_q0 = bp->primitive_pairs_locs[cp_q];
-        tk.nprim_p = bp->cptype[cp_p].nprim_12;
-        tk.nprim_q = bp->cptype[cp_q].nprim_12;
-        tk.diag = c.diag;
-        tk.nbins = nbins;
-        for (int b = 0; b <= nbins; ++b) {
-            tk.pbins[b] = pbins[b];
-            tk.qbins[b] = qbins[b];
-        }
-    }
-    size_t off_p = stage.size();
-    side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], stage);
-    size_t off_q = off_p;
-    if (cp_p != cp_q) {
-        off_q = stage.size();
-        side_order(qbins, nbins, bra_order + locs[cp_q], locs[cp_q + 1] - locs[cp_q], stage);
-    }
-    std::vector<int> items[2];
-    for (int p = 0; p < np; ++p) {
+    const int ck[2] = {tile_for(wave ? 4 : 1, cls, 0), tile_for(wave ? 4 : 1, cls, 1)};
+    const int step = wave ? 1 : 64;
+    for (int p = 0; p < np; p += step) {
         int s = (int)(std::upper_bound(pbins + 1, pbins + nbins + 1, p) - (pbins + 1));
         int qmax = qbins[nbins - s];
         double ubp = bin_ub(c, cp_p, pbins, s);
@@ -513,25 +418,161 @@ static int plan_wave(const TaskCtx &c, const int *bra_order, std::vector<int> &s
             }
         int qsplit = qbins[split];
         int lim[2][2] = {{0, qsplit < qmax ? qsplit : qmax}, {qsplit, qmax}};
-        for (int prec = 0; prec < 2; ++prec)
-            for (int q0 = lim[prec][0]; q0 < lim[prec][1]; q0 += BQC_WAVE_CK) {
-                int q1 = q0 + BQC_WAVE_CK < lim[prec][1] ? q0 + BQC_WAVE_CK : lim[prec][1];
-                items[prec].push_back(p);
-                items[prec].push_back(q0);
-                items[prec].push_back(q1);
-                items[prec].push_back(0);
+        const int pitem = wave ? p : p / 64;
+        for (int prec = 0; prec < 2; ++prec) {
+            std::vector<int> &it = g[prec]->items;
+            for (int q0 = lim[prec][0]; q0 < lim[prec][1]; q0 += ck[prec]) {
+                int q1 = q0 + ck[prec] < lim[prec][1] ? q0 + ck[prec] : lim[prec][1];
+                it.push_back(ti);
+                it.push_back(pitem);
+                it.push_back(q0);
+                it.push_back(q1);
             }
+        }
     }
-    for (int prec = 0; prec < 2; ++prec) {
-        int nitems = (int)items[prec].size() / 4;
+    return 0;
+}
+
+// Device image of a BuildPlan: [TaskDesc array][list orders][items].
+struct Staged {
+    size_t bytes = 0, off_perm = 0;
+    std::vector<size_t> off_items;  // per group
+};
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+static Staged stage_layout(const BuildPlan &p) {
+    Staged st;
+    size_t off = align16(p.tasks.size() * sizeof(TaskDesc));
+    st.off_perm = off;
+    off = align16(off + p.ints.size() * sizeof(int));
+    for (const Group &g : p.groups) {
+        st.off_items.push_back(off);
+        off = align16(off + g.items.size() * sizeof(int));
+    }
+    st.bytes = off;
+    return st;
+}
+
+// Writes the image for device address dev into host buffer host.
+static void stage_fill(BuildPlan &p, const Staged &st, char *dev, char *host) {
+    int *perm_dev = (int *)(dev + st.off_perm);
+    for (size_t t = 0; t < p.tasks.size(); ++t) {
+        p.tasks[t].perm_p = perm_dev + p.perm_p_off[t];
+        p.tasks[t].perm_q = perm_dev + p.perm_q_off[t];
+    }
+    memcpy(host, p.tasks.data(), p.tasks.size() * sizeof(TaskDesc));
+    memcpy(host + st.off_perm, p.ints.data(), p.ints.size() * sizeof(int));
+    for (size_t gi = 0; gi < p.groups.size(); ++gi)
+        memcpy(host + st.off_items[gi], p.groups[gi].items.data(),
+               p.groups[gi].items.size() * sizeof(int));
+}
+
+static int launch_groups(const BuildPlan &p, const Staged &st, char *dev, hipStream_t stream) {
+    const TaskDesc *tasks = (const TaskDesc *)dev;
+    for (size_t gi = 0; gi < p.groups.size(); ++gi) {
+        const Group &g = p.groups[gi];
+        int nitems = (int)g.items.size() / 4;
         if (!nitems) continue;
-        while (stage.size() & 3) stage.push_back(0);
-        r[prec].off_perm_p = off_p;
-        r[prec].off_perm_q = off_q;
-        r[prec].off_items = stage.size();
-        stage.insert(stage.end(), items[prec].begin(), items[prec].end());
-        r[prec].task.nitems = nitems;
-        rows.push_back(r[prec]);
+        const int4 *items = (const int4 *)(dev + st.off_items[gi]);
+        unsigned nblk = g.wave ? (unsigned)nitems : (unsigned)((nitems + 3) / 4);
+        unsigned nthr = g.wave ? 64 : 256;
+        LaunchRec rec;
+        if (g_prof) record_launch(rec, g.cls, (g.wave ? 4 : 1) + 2 * g.prec, g.flops, g.nquart, stream);
+        hipLaunchKernelGGL(g.kern, dim3(nblk), dim3(nthr), 0, stream, tasks, items, nitems);
+        if (g_prof) {
+            hipEventRecord(rec.e1, stream);
+            g_recs.push_back(rec);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            fprintf(stderr, "bqcjk: kernel launch failed: %s\n", hipGetErrorString(e));
+            return 1;
+        }
+    }
+    return 0;
+}
+
+// Shared device image for direct (non-graph) builds.
+static char *g_dws = nullptr;
+static size_t g_dws_cap = 0;
+static char *g_hstage = nullptr;
+static size_t g_hstage_cap = 0;
+static hipEvent_t g_stage_evt = nullptr;
+static bool g_stage_pending = false;
+
+static int stage_and_launch(BuildPlan &p, hipStream_t stream) {
+    if (p.groups.empty()) return 0;
+    Staged st = stage_layout(p);
+    if (g_stage_pending) {
+        hipEventSynchronize(g_stage_evt);
+        g_stage_pending = false;
+    }
+    if (st.bytes > g_hstage_cap) {
+        if (g_hstage) hipHostFree(g_hstage);
+        size_t cap = st.bytes + st.bytes / 2 + 4096;
+        if (hipHostMalloc((void **)&g_hstage, cap, hipHostMallocDefault) != hipSuccess) return 1;
+        g_hstage_cap = cap;
+    }
+    if (st.bytes > g_dws_cap) {
+        // kernels queued earlier may still read the old image
+        hipStreamSynchronize(stream);
+        if (g_dws) hipFree(g_dws);
+        size_t cap = st.bytes + st.bytes / 2 + 4096;
+        if (hipMalloc((void **)&g_dws, cap) != hipSuccess) return 1;
+        g_dws_cap = cap;
+    }
+    if (!g_stage_evt) hipEventCreateWithFlags(&g_stage_evt, hipEventDisableTiming);
+    stage_fill(p, st, g_dws, g_hstage);
+    if (hipMemcpyAsync(g_dws, g_hstage, st.bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
+        return 1;
+    hipEventRecord(g_stage_evt, stream);
+    g_stage_pending = true;
+    return launch_groups(p, st, g_dws, stream);
+}
+
+struct MolArgs {
+    BasisProdCache *bp;
+    double *vj, *vk, *dm;
+    int nao, n_dm, ntask;
+    const int *task_cp, *task_bins, *bra_order;
+    const double *log_q, *task_dmax;
+};
+
+struct Deferred {
+    TaskCtx c;
+    JKArgs jk;
+    PairData pd;
+};
+
+// Plans all tasks of all molecules.  Classes without row/wave kernels go to
+// the generic kernels: launched right away, or queued in *deferred.
+static int plan_molecules(BuildPlan &plan, const MolArgs *m, int nmol, int nbins,
+                          int precision, double fp32_cut, hipStream_t stream,
+                          std::vector<Deferred> *deferred = nullptr) {
+    if (precision != 0 && precision != 1) return 1;
+    if (nbins > BQC_MAX_BINS) return 3;
+    register_kernels();
+    for (int im = 0; im < nmol; ++im) {
+        const MolArgs &a = m[im];
+        JKArgs jk = {a.vj, a.vk, a.dm, a.nao, a.n_dm};
+        PairData pd = make_pair_data(a.bp);
+        for (int t = 0; t < a.ntask; ++t) {
+            const int *bi = a.task_bins + (size_t)t * 2 * (nbins + 1);
+            const int *bk = bi + nbins + 1;
+            TaskCtx c;
+            int err = task_init(c, a.bp, bi, bk, nbins, a.task_cp[2 * t], a.task_cp[2 * t + 1],
+                                a.log_q, a.task_dmax ? a.task_dmax[t] : 0, fp32_cut, precision);
+            if (err) return err;
+            err = plan_task(plan, c, 0, a.bra_order, jk, pd);
+            if (err < 0) err = plan_task(plan, c, 1, a.bra_order, jk, pd);
+            if (err < 0 && deferred) {
+                deferred->push_back(Deferred{c, jk, pd});
+                err = 0;
+            }
+            if (err < 0) err = launch_quartets(c, jk, pd, stream, plan.flops);
+            if (err > 0) return err;
+        }
     }
     return 0;
 }
@@ -556,52 +597,133 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
                 int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
                 const int *bra_order, int precision, const double *log_q,
                 const double *task_dmax, double fp32_cut, hipStream_t stream) {
-    if (precision != 0 && precision != 1) return 1;
-    if (nbins > BQC_MAX_BINS) return 3;
-    register_kernels();
-    JKArgs jk = {vj, vk, dm, nao, n_dm};
-    PairData pd = make_pair_data(bpcache);
-    double flops[2] = {0, 0};
-    std::vector<RowLaunch> rows;
-    std::vector<int> stage;
-    for (int t = 0; t < ntask; ++t) {
-        const int *bi = task_bins + (size_t)t * 2 * (nbins + 1);
-        const int *bk = bi + nbins + 1;
-        TaskCtx c;
-        int err = task_init(c, bpcache, bi, bk, nbins, task_cp[2 * t], task_cp[2 * t + 1], log_q,
-                            task_dmax ? task_dmax[t] : 0, fp32_cut, precision);
-        if (err) return err;
-        err = plan_rows(c, bra_order, stage, rows, flops);
-        if (err < 0) err = plan_wave(c, bra_order, stage, rows, flops);
-        if (err < 0) err = launch_quartets(c, jk, pd, stream, flops);
-        if (err > 0) return err;
-    }
-    if (!rows.empty()) {
-        int err = stage_to_device(stage, stream);
-        if (err) return err;
-        for (RowLaunch &r : rows) {
-            r.task.perm_p = g_dws + r.off_perm_p;
-            r.task.perm_q = g_dws + r.off_perm_q;
-            r.task.items = (const int4 *)(g_dws + r.off_items);
-            unsigned nblk = r.wave ? (unsigned)r.task.nitems : (unsigned)((r.task.nitems + 3) / 4);
-            unsigned nthr = r.wave ? 64 : 256;
-            LaunchRec rec;
-            if (g_prof)
-                record_launch(rec, r.cls, (r.wave ? 4 : 1) + 2 * r.prec, r.flops, r.nquart, stream);
-            hipLaunchKernelGGL(r.kern, dim3(nblk), dim3(nthr), 0, stream, jk, pd, r.task);
-            if (g_prof) {
-                hipEventRecord(rec.e1, stream);
-                g_recs.push_back(rec);
-            }
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) {
-                fprintf(stderr, "bqcjk: row kernel launch failed: %s\n", hipGetErrorString(e));
-                return 1;
-            }
+    MolArgs m = {bpcache, vj, vk, dm, nao, n_dm, ntask, task_cp, task_bins, bra_order,
+                 log_q, task_dmax};
+    BuildPlan plan;
+    int err = plan_molecules(plan, &m, 1, nbins, precision, fp32_cut, stream);
+    if (!err) err = stage_and_launch(plan, stream);
+    g_last_flops[0] = plan.flops[0];
+    g_last_flops[1] = plan.flops[1];
+    return err;
+}
+
+int BQCbuild_jk_batch(int nmol, BasisProdCache **bps, double **vj, double **vk, double **dm,
+                      const int *nao, int n_dm, const int *ntask, const int **task_cp,
+                      const int **task_bins, int nbins, const int **bra_order, int precision,
+                      const double **log_q, const double **task_dmax, double fp32_cut,
+                      hipStream_t stream) {
+    std::vector<MolArgs> m(nmol);
+    for (int i = 0; i < nmol; ++i)
+        m[i] = MolArgs{bps[i], vj ? vj[i] : nullptr, vk ? vk[i] : nullptr, dm[i], nao[i], n_dm,
+                       ntask[i], task_cp[i], task_bins[i], bra_order[i],
+                       log_q ? log_q[i] : nullptr, task_dmax ? task_dmax[i] : nullptr};
+    BuildPlan plan;
+    int err = plan_molecules(plan, m.data(), nmol, nbins, precision, fp32_cut, stream);
+    if (!err) err = stage_and_launch(plan, stream);
+    g_last_flops[0] = plan.flops[0];
+    g_last_flops[1] = plan.flops[1];
+    return err;
+}
+
+// A captured batch: its own device image and the instantiated graph.
+struct BQCGraph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    char *dev;
+    double flops[2];
+};
+
+void *BQCgraph_capture(int nmol, BasisProdCache **bps, double **vj, double **vk, double **dm,
+                       const int *nao, int n_dm, const int *ntask, const int **task_cp,
+                       const int **task_bins, int nbins, const int **bra_order, int precision,
+                       const double **log_q, const double **task_dmax, double fp32_cut,
+                       hipStream_t stream) {
+    std::vector<MolArgs> m(nmol);
+    for (int i = 0; i < nmol; ++i)
+        m[i] = MolArgs{bps[i], vj ? vj[i] : nullptr, vk ? vk[i] : nullptr, dm[i], nao[i], n_dm,
+                       ntask[i], task_cp[i], task_bins[i], bra_order[i],
+                       log_q ? log_q[i] : nullptr, task_dmax ? task_dmax[i] : nullptr};
+    BuildPlan plan;
+    std::vector<Deferred> deferred;
+    bool was_prof = g_prof;
+    g_prof = false;
+    int err = plan_molecules(plan, m.data(), nmol, nbins, precision, fp32_cut, stream, &deferred);
+    BQCGraph *g = new BQCGraph();
+    g->dev = nullptr;
+    g->graph = nullptr;
+    Staged st;
+    if (!err && !plan.groups.empty()) {
+        st = stage_layout(plan);
+        std::vector<char> host(st.bytes);
+        if (hipMalloc((void **)&g->dev, st.bytes) != hipSuccess) err = 1;
+        if (!err) {
+            stage_fill(plan, st, g->dev, host.data());
+            if (hipMemcpy(g->dev, host.data(), st.bytes, hipMemcpyHostToDevice) != hipSuccess)
+                err = 1;
         }
     }
-    g_last_flops[0] = flops[0];
-    g_last_flops[1] = flops[1];
+    hipStream_t cs = nullptr;
+    if (!err && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) err = 1;
+    if (!err && hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 1;
+    if (!err) {
+        double fl[2] = {0, 0};
+        for (Deferred &d : deferred)
+            if (!err) err = launch_quartets(d.c, d.jk, d.pd, cs, fl);
+        plan.flops[0] += fl[0];
+        plan.flops[1] += fl[1];
+        if (!err && !plan.groups.empty()) err = launch_groups(plan, st, g->dev, cs);
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(cs, &graph);
+        if (ce != hipSuccess || !graph) err = 1;
+        g->graph = graph;
+    }
+    if (!err && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) err = 1;
+    g_prof = was_prof;
+    if (cs) hipStreamDestroy(cs);
+    if (err) {
+        if (g->graph) hipGraphDestroy(g->graph);
+        if (g->dev) hipFree(g->dev);
+        delete g;
+        return nullptr;
+    }
+    g->flops[0] = plan.flops[0];
+    g->flops[1] = plan.flops[1];
+    return g;
+}
+
+int BQCgraph_launch(void *handle, hipStream_t stream) {
+    BQCGraph *g = (BQCGraph *)handle;
+    if (!g) return 1;
+    g_last_flops[0] = g->flops[0];
+    g_last_flops[1] = g->flops[1];
+    return hipGraphLaunch(g->exec, stream) == hipSuccess ? 0 : 1;
+}
+
+void BQCgraph_destroy(void *handle) {
+    BQCGraph *g = (BQCGraph *)handle;
+    if (!g) return;
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    if (g->dev) hipFree(g->dev);
+    delete g;
+}
+
+// Q pairs per work item: kind 1 row kernels, 4 wave kernels; cls = class code
+// li*1000+lj*100+lk*10+ll as evaluated, or -1 for the kind's default.
+int BQCset_tile(int kind, int cls, int prec, int ck) {
+    if (ck < 1 || ck > 4096 || (kind != 1 && kind != 4)) return 1;
+    if (cls < 0) {
+        if (kind == 1) g_ck_rows = ck;
+        else g_ck_wave = ck;
+        return 0;
+    }
+    long long key = ((long long)kind * 10000 + cls) * 2 + prec;
+    for (auto &kv : g_ck_override)
+        if (kv.first == key) {
+            kv.second = ck;
+            return 0;
+        }
+    g_ck_override.push_back({key, ck});
     return 0;
 }
 

@@ -30,7 +30,7 @@
 
 using namespace bqc;
 
-#define BQC_MAX_BINS 64
+#define BQC_MAX_BINS 32
 #define BQC_THREADS 256
 
 struct TaskArgs {
@@ -58,6 +58,23 @@ struct PairData {
     const double *a12, *e12, *x12, *y12, *z12;
     const double *bas_x, *bas_y, *bas_z;
     const int *bra, *ket, *ao_loc;
+};
+
+// Everything a work item needs about its task, in device memory: the pair
+// data of the task's plan, the list orders of both sides, the density and
+// J/K matrices and the Schwarz bins.  Items of many tasks -- of one molecule
+// or of many -- can then share one launch.
+struct TaskDesc {
+    PairData pd;
+    const int *perm_p, *perm_q;  // list position -> plan position
+    const double *dm;
+    double *vj, *vk;
+    int nao, n_dm;
+    int np;                      // P pairs in bins
+    int bas_p0, bas_q0, prim_p0, prim_q0, nprim_p, nprim_q;
+    int diag, nbins;
+    int pbins[BQC_MAX_BINS + 1];
+    int qbins[BQC_MAX_BINS + 1];
 };
 
 __device__ __forceinline__ void atomic_add(double *p, double v) {

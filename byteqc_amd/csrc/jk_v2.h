@@ -22,19 +22,6 @@
 
 #define BQC_V2_CK 64
 
-struct V2Task {
-    const int *perm_p;  // list position -> plan position, P side
-    const int *perm_q;
-    const int4 *items;  // (chunk, q0, q1, -)
-    int nitems;
-    int np;             // P pairs in bins
-    int bas_p0, bas_q0, prim_p0, prim_q0, nprim_p, nprim_q;
-    int diag;
-    int nbins;
-    int pbins[BQC_MAX_BINS + 1];
-    int qbins[BQC_MAX_BINS + 1];
-};
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -52,17 +39,22 @@ __device__ __forceinline__ double run_sum(double v, int lane, int run_end) {
     return v;
 }
 
+// Work item: (task, chunk of 64 P pairs, first and end Q position).
 template <int LA, int LB, int LC, int LD, typename T>
-__global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk) {
+__global__ void __launch_bounds__(256)
+    jk_rows(const TaskDesc *__restrict__ tasks, const int4 *__restrict__ items, int nitems) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NA = C::NFI, NB = C::NFJ, NC = C::NFK, ND = C::NFL;
     const int lane = threadIdx.x & 63;
     const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (item >= tk.nitems) return;
-    const int4 it = tk.items[item];
-    const int chunk = __builtin_amdgcn_readfirstlane(it.x);
-    const int q0 = __builtin_amdgcn_readfirstlane(it.y);
-    const int qe = __builtin_amdgcn_readfirstlane(it.z);
+    if (item >= nitems) return;
+    const int4 it = items[item];
+    const TaskDesc &tk = tasks[__builtin_amdgcn_readfirstlane(it.x)];
+    const PairData &pd = tk.pd;
+    const JKArgs jk = {tk.vj, tk.vk, tk.dm, tk.nao, tk.n_dm};
+    const int chunk = __builtin_amdgcn_readfirstlane(it.y);
+    const int q0 = __builtin_amdgcn_readfirstlane(it.z);
+    const int qe = __builtin_amdgcn_readfirstlane(it.w);
     const int pidx = chunk * 64 + lane;
     const bool pvalid = pidx < tk.np;
     // bin of this P pair -> end of its Q prefix
@@ -264,7 +256,7 @@ __global__ void __launch_bounds__(256) jk_rows(JKArgs jk, PairData pd, V2Task tk
     }
 }
 
-typedef void (*RowKernel)(JKArgs, PairData, V2Task);
+typedef void (*RowKernel)(const TaskDesc *, const int4 *, int);
 typedef RowKernel RowKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
 
 template <int LA, int LB, int LC, int LD>

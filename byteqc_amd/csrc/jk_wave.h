@@ -38,8 +38,10 @@ __device__ __forceinline__ int gidx(int n, int d) {
     return C::g(e(LA, x), e(LB, y), e(LC, z), e(LD, w));
 }
 
+// Work item: (task, P position, first and end Q position).
 template <int LA, int LB, int LC, int LD, typename T>
-__global__ void __launch_bounds__(64) jk_wave(JKArgs jk, PairData pd, V2Task tk) {
+__global__ void __launch_bounds__(64)
+    jk_wave(const TaskDesc *__restrict__ tasks, const int4 *__restrict__ items, int nitems) {
     using W = WaveCls<LA, LB, LC, LD>;
     using C = typename W::C;
     constexpr int NR = C::NR, NA = W::NA, NB = W::NB, NC = W::NC, ND = W::ND;
@@ -51,13 +53,16 @@ __global__ void __launch_bounds__(64) jk_wave(JKArgs jk, PairData pd, V2Task tk)
 
     const int lane = threadIdx.x;
     const int item = blockIdx.x;
-    if (item >= tk.nitems) return;
-    const int4 it = tk.items[item];
-    const int pidx = it.x;
+    if (item >= nitems) return;
+    const int4 it = items[item];
+    const TaskDesc &tk = tasks[it.x];
+    const PairData &pd = tk.pd;
+    const JKArgs jk = {tk.vj, tk.vk, tk.dm, tk.nao, tk.n_dm};
+    const int pidx = it.y;
     int s = 0;
     while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
-    const int q0 = it.y;
-    const int q1 = min(it.z, tk.qbins[tk.nbins - s]);
+    const int q0 = it.z;
+    const int q1 = min(it.w, tk.qbins[tk.nbins - s]);
     if (q1 <= q0) return;
     const int ppos = tk.perm_p[pidx];
     const int gp = tk.bas_p0 + ppos;

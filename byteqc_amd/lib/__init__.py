@@ -29,6 +29,18 @@ _SIGNATURES = {
                           c_void_p, c_void_p]),
     'BQCint1e': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                          c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    'BQCbuild_jk_batch': (c_int, [c_int] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p,
+                                  c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                  c_double, c_void_p]),
+    'BQCgraph_capture': (c_void_p, [c_int] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p,
+                                    c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                    c_double, c_void_p]),
+    'BQCgraph_launch': (c_int, [c_void_p, c_void_p]),
+    'BQCgraph_destroy': (None, [c_void_p]),
+    'BQCset_tile': (c_int, [c_int, c_int, c_int, c_int]),
+    'BQCplan_tasks': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double,
+                              c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'BQCdm_cond': (c_int, [c_void_p] * 7 + [c_int, c_int, c_void_p]),
     'BQCsymmetrize_jk': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
     'BQClast_build_flops': (None, [c_void_p]),
     'BQCprofile_enable': (None, [c_int]),

@@ -274,7 +274,7 @@ class _VHFOpt:
         q_cond = q_d.cpu().numpy()
         self.q_cond = q_cond
 
-        log_qs, bra, ket, cp_l = [], [], [], []
+        log_qs, bra, ket, cp_l, cp_grp = [], [], [], [], []
         offs = self.l_ctr_offsets
         for i in range(len(uniq_l_ctr)):
             p0, p1 = offs[i], offs[i + 1]
@@ -293,8 +293,10 @@ class _VHFOpt:
                 lq[lq > 0] = 0
                 log_qs.append(lq)
                 cp_l.append((int(uniq_l_ctr[i, 0]), int(uniq_l_ctr[j, 0])))
+                cp_grp.append((i, j))
         self.log_qs = log_qs
         self.cp_l = cp_l
+        self.cp_grp = numpy.ascontiguousarray(numpy.asarray(cp_grp, dtype=numpy.int32).reshape(-1, 2))
         self.bas_pair2shls = numpy.ascontiguousarray(
             numpy.vstack([numpy.hstack(bra), numpy.hstack(ket)]).astype(numpy.int32))
         self.bas_pairs_locs = numpy.append(0, numpy.cumsum([len(x) for x in bra])).astype(
@@ -358,32 +360,52 @@ class _VHFOpt:
 
     # -- per-build task list ------------------------------------------------
     def tasks(self, dm_cond, nbins, prescreen=True):
-        """(task_cp, task_bins, task_dmax) of one build, as hf.py:111-166."""
+        """(task_cp, task_bins, task_dmax) of one build, as hf.py:111-166
+        (computed by BQCplan_tasks)."""
         ncp = len(self.log_qs)
-        cp_i, cp_j = numpy.tril_indices(len(self.uniq_l_ctr))
-        tasks, bins, dmax = [], [], []
-        for t in range(ncp * (ncp + 1) // 2):
-            a = int(numpy.floor(numpy.sqrt(2 * t)))
-            if a * (a + 1) // 2 > t:
-                a -= 1
-            b = t - a * (a + 1) // 2
-            lqa, lqb = self.log_qs[a], self.log_qs[b]
-            if lqa.size == 0 or lqb.size == 0:
-                continue
-            i, j, k, l = cp_i[a], cp_j[a], cp_i[b], cp_j[b]
-            sub = max(dm_cond[i, j], dm_cond[k, l], dm_cond[i, k], dm_cond[j, k],
-                      dm_cond[i, l], dm_cond[j, l])
-            if prescreen and sub < self.direct_scf_tol:
-                continue
-            cut = self.direct_scf_tol / sub if sub > 0 else 1e300
-            tasks.append((a, b))
-            bins.append(numpy.concatenate([_make_s_index_offsets(lqa, nbins, cut),
-                                           _make_s_index_offsets(lqb, nbins, cut)]))
-            dmax.append(sub)
-        task_cp = numpy.ascontiguousarray(numpy.asarray(tasks, dtype=numpy.int32).reshape(-1, 2))
-        task_bins = numpy.ascontiguousarray(
-            numpy.asarray(bins, dtype=numpy.int32).reshape(len(tasks), 2 * (nbins + 1)))
-        return task_cp, task_bins, numpy.asarray(dmax, dtype=numpy.float64)
+        nmax = ncp * (ncp + 1) // 2
+        task_cp = numpy.zeros((max(nmax, 1), 2), dtype=numpy.int32)
+        task_bins = numpy.zeros((max(nmax, 1), 2 * (nbins + 1)), dtype=numpy.int32)
+        task_dmax = numpy.zeros(max(nmax, 1))
+        nt = ctypes.c_int(0)
+        dmc = numpy.ascontiguousarray(dm_cond, dtype=numpy.float64)
+        load_library().BQCplan_tasks(ncp, _ptr(self.cp_grp), _ptr(self.bas_pairs_locs),
+                                     _ptr(self.log_q), _ptr(dmc), len(self.uniq_l_ctr),
+                                     self.direct_scf_tol, nbins, 1 if prescreen else 0,
+                                     _ptr(task_cp), _ptr(task_bins), _ptr(task_dmax),
+                                     ctypes.byref(nt))
+        n = nt.value
+        return task_cp[:n], task_bins[:n], task_dmax[:n]
+
+    def default_nbins(self):
+        pairs_max = int(numpy.diff(self.bas_pairs_locs).max())
+        return max(10, pairs_max // 100000)
+
+    def group_ao_loc(self):
+        return numpy.ascontiguousarray(self.mol.ao_loc_nr()[self.l_ctr_offsets], dtype=numpy.int32)
+
+
+def dm_cond_batch(dmcs, locs, stream=None):
+    """Block bounds max|D| over (group, group) AO blocks for device matrices
+    dmcs[m] (nao_m x nao_m) with group AO offsets locs[m]; returns host arrays."""
+    dev = dmcs[0].device
+    ng = numpy.array([len(l) - 1 for l in locs], dtype=numpy.int32)
+    off = numpy.append(0, numpy.cumsum([len(l) for l in locs]))[:-1].astype(numpy.int32)
+    outoff = numpy.append(0, numpy.cumsum(ng * ng))[:-1].astype(numpy.int32)
+    ptrs = torch.tensor([m.data_ptr() for m in dmcs], dtype=torch.int64, device=dev)
+    nao = torch.tensor([m.shape[-1] for m in dmcs], dtype=torch.int32, device=dev)
+    loc_d = torch.as_tensor(numpy.concatenate(locs).astype(numpy.int32), device=dev)
+    off_d = torch.as_tensor(off, device=dev)
+    ng_d = torch.as_tensor(ng, device=dev)
+    oo_d = torch.as_tensor(outoff, device=dev)
+    out = torch.zeros(int((ng * ng).sum()), dtype=torch.float64, device=dev)
+    err = load_library().BQCdm_cond(_ptr(ptrs), _ptr(nao), _ptr(loc_d), _ptr(off_d), _ptr(ng_d),
+                                    _ptr(oo_d), _ptr(out), len(dmcs), int(ng.max()),
+                                    _stream_ptr(stream))
+    if err:
+        raise RuntimeError('density bound kernel failed')
+    host = out.cpu().numpy()
+    return [host[o:o + g * g].reshape(g, g) for o, g in zip(outoff, ng)]
 
 
 def _condense_absmax(d, loc):
@@ -425,10 +447,9 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None,
     n_dm = dms.shape[0]
     dmc = torch.matmul(torch.matmul(coeff, dms), coeff.T).contiguous()
     if nbins is None:
-        pairs_max = int(numpy.diff(vhfopt.bas_pairs_locs).max())
-        nbins = max(10, pairs_max // 100000)
-    loc = vhfopt.mol.ao_loc_nr()[vhfopt.l_ctr_offsets]
-    dm_cond = numpy.max([_condense_absmax(x, loc) for x in dmc], axis=0)
+        nbins = vhfopt.default_nbins()
+    loc = vhfopt.group_ao_loc()
+    dm_cond = numpy.max(dm_cond_batch(list(dmc), [loc] * n_dm, stream), axis=0)
     task_cp, task_bins, task_dmax = vhfopt.tasks(dm_cond, nbins, prescreen)
     vj = torch.zeros((n_dm, nao, nao), dtype=torch.float64, device=dev) if with_j else None
     vk = torch.zeros((n_dm, nao, nao), dtype=torch.float64, device=dev) if with_k else None

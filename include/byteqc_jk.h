@@ -115,6 +115,56 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
                 const int *bra_order, int precision, const double *log_q,
                 const double *task_dmax, double fp32_cut, hipStream_t stream);
 
+/* The same build for many independent molecules in one set of launches
+ * (one launch per integral class and kernel kind across all molecules).
+ * Arrays of length nmol hold each molecule's plan, device J/K/density
+ * pointers (vj or vk may be NULL as a whole), nao, task list and first-shell
+ * order exactly as for BQCbuild_jk; log_q / task_dmax may be NULL for FP64.
+ * Returns as BQCbuild_jk. */
+int BQCbuild_jk_batch(int nmol, BasisProdCache **bps, double **vj, double **vk, double **dm,
+                      const int *nao, int n_dm, const int *ntask, const int **task_cp,
+                      const int **task_bins, int nbins, const int **bra_order, int precision,
+                      const double **log_q, const double **task_dmax, double fp32_cut,
+                      hipStream_t stream);
+
+/* Captures the launches of a BQCbuild_jk_batch call into a HIP graph with
+ * its own device work list (the density and J/K buffers passed here are the
+ * ones the graph reads and accumulates into on every replay; the caller
+ * zeroes J/K before a replay with a kernel, not a memset).  Returns an
+ * opaque handle, NULL on error. */
+void *BQCgraph_capture(int nmol, BasisProdCache **bps, double **vj, double **vk, double **dm,
+                       const int *nao, int n_dm, const int *ntask, const int **task_cp,
+                       const int **task_bins, int nbins, const int **bra_order, int precision,
+                       const double **log_q, const double **task_dmax, double fp32_cut,
+                       hipStream_t stream);
+int BQCgraph_launch(void *graph, hipStream_t stream);
+void BQCgraph_destroy(void *graph);
+
+/* Q pairs per work item of the row kernels (kind 1) or wave kernels
+ * (kind 4), for class code cls (li*1000+lj*100+lk*10+ll as evaluated) and
+ * precision prec (0 FP64, 1 FP32), or for all classes of the kind (cls < 0).
+ * Used by the tile autotuner.  Returns 0 ok, 1 bad argument. */
+int BQCset_tile(int kind, int cls, int prec, int ck);
+
+/* Task list of one build on the host (the per-task loop of
+ * cuobc/scf/hf.py:111-145 with _make_s_index_offsets, hf.py:567-578).
+ * cp_grp (2 x ncp): group pair (gi, gj) of each pair type; log_q (plan
+ * order) and bas_pairs_locs as in the plan; dm_cond (ngroups^2): block
+ * bounds of |D|.  Writes task_cp (2 per task), task_bins (2 x (nbins+1) per
+ * task) and task_dmax; the output arrays hold ncp(ncp+1)/2 tasks. */
+int BQCplan_tasks(int ncp, const int *cp_grp, const int *bas_pairs_locs, const double *log_q,
+                  const double *dm_cond, int ngroups, double cutoff, int nbins, int prescreen,
+                  int *task_cp, int *task_bins, double *task_dmax, int *ntask_out);
+
+/* Block-wise max |D| of nmat device matrices over their (group, group) AO
+ * blocks.  mats (device array of device pointers), nao, off, ng, outoff:
+ * device int arrays per matrix; loc (device): AO offsets of the groups of
+ * matrix m at loc[off[m] .. off[m]+ng[m]]; out (device) receives ng[m]^2
+ * values per matrix at outoff[m]; gmax = max ng (<= 64). */
+int BQCdm_cond(const double *const *mats, const int *nao, const int *loc, const int *off,
+               const int *ng, const int *outoff, double *out, int nmat, int gmax,
+               hipStream_t stream);
+
 /* Schwarz bounds on the GPU: q[i*nbas + j] = sqrt(max |(ij|ij)|) over the
  * Cartesian components of shells i, j (1e-100 when zero), for the shell
  * groups [grp_offsets[g], grp_offsets[g+1]) (ngroups groups, host array);

@@ -269,6 +269,36 @@ def _condense_absmax(d, loc):
     return out
 
 
+def task_list(plan, cond, nbins, prescreen=True):
+    """Tasks (pair type a >= b), their bin offsets and density bounds, as the
+    reference's loop in cuobc/scf/hf.py:111-145 (bounds: hf.py:119-126, bins:
+    _make_s_index_offsets, hf.py:567-578)."""
+    ncp = len(plan.log_qs)
+    cp_i, cp_j = numpy.tril_indices(len(plan.uniq_l_ctr))
+    tasks, bins, subs = [], [], []
+    for t in range(ncp * (ncp + 1) // 2):
+        a = int(numpy.floor(numpy.sqrt(2 * t)))
+        if a * (a + 1) // 2 > t:
+            a -= 1
+        b = t - a * (a + 1) // 2
+        lqa, lqb = plan.log_qs[a], plan.log_qs[b]
+        if lqa.size == 0 or lqb.size == 0:
+            continue
+        i, j, k, l = cp_i[a], cp_j[a], cp_i[b], cp_j[b]
+        sub = max(cond[i, j], cond[k, l], cond[i, k], cond[j, k], cond[i, l], cond[j, l])
+        if prescreen and sub < plan.cutoff:
+            continue
+        with numpy.errstate(divide='ignore'):
+            cut = plan.cutoff / sub
+        tasks.append((a, b))
+        subs.append(sub)
+        bins.append(numpy.concatenate([make_s_index_offsets(lqa, nbins, cut),
+                                       make_s_index_offsets(lqb, nbins, cut)]))
+    task_cp = numpy.asarray(tasks, dtype=numpy.int32).reshape(-1, 2)
+    task_bins = numpy.asarray(bins, dtype=numpy.int32).reshape(len(tasks), 2 * (nbins + 1))
+    return task_cp, task_bins, numpy.asarray(subs)
+
+
 def get_jk(mol, dm, plan=None, with_j=True, with_k=True, prescreen=True, nbins=None,
            return_count=False, kl_stride=1):
     """Oracle J and K for hermitian densities dm of shape (nao0,nao0) or (n,nao0,nao0)."""
@@ -283,29 +313,10 @@ def get_jk(mol, dm, plan=None, with_j=True, with_k=True, prescreen=True, nbins=N
     if nbins is None:
         pairs_max = int(numpy.diff(plan.bas_pairs_locs).max())
         nbins = max(10, pairs_max // 100000)
-    ncp = len(plan.log_qs)
-    cp_i, cp_j = numpy.tril_indices(len(plan.uniq_l_ctr))
     loc = plan.ao_loc[plan.l_ctr_offsets]
     cond = numpy.max([_condense_absmax(x, loc) for x in dmc], axis=0)
-    tasks, bins = [], []
-    for t in range(ncp * (ncp + 1) // 2):
-        a = int(numpy.floor(numpy.sqrt(2 * t)))
-        if a * (a + 1) // 2 > t:
-            a -= 1
-        b = t - a * (a + 1) // 2
-        lqa, lqb = plan.log_qs[a], plan.log_qs[b]
-        if lqa.size == 0 or lqb.size == 0:
-            continue
-        i, j, k, l = cp_i[a], cp_j[a], cp_i[b], cp_j[b]
-        sub = max(cond[i, j], cond[k, l], cond[i, k], cond[j, k], cond[i, l], cond[j, l])
-        if prescreen and sub < plan.cutoff:
-            continue
-        cut = plan.cutoff / sub
-        tasks.append((a, b))
-        bins.append(numpy.concatenate([make_s_index_offsets(lqa, nbins, cut),
-                                       make_s_index_offsets(lqb, nbins, cut)]))
-    task_cp = numpy.asarray(tasks, dtype=numpy.int32).reshape(-1, 2)
-    task_bins = numpy.asarray(bins, dtype=numpy.int32).reshape(len(tasks), -1)
+    task_cp, task_bins, _ = task_list(plan, cond, nbins, prescreen)
+    tasks = task_cp
     vj = numpy.zeros((n_dm, nao, nao))
     vk = numpy.zeros((n_dm, nao, nao))
     nq = ctypes.c_longlong(0)

@@ -150,3 +150,44 @@ def test_workload_generator_is_seeded_and_respects_min_distance():
     assert numpy.allclose(numpy.linalg.norm(h - o[:, None], axis=-1), 0.9572)
     dm = random_density(30, 5, 1)
     assert numpy.allclose(dm @ dm, 2 * dm)
+
+
+@pytest.mark.parametrize('prescreen', [True, False])
+def test_task_planner_matches_oracle(prescreen):
+    """BQCplan_tasks (C++) gives the reference's task list, bins and bounds."""
+    from byteqc_amd.lib import library_path, load_library
+    if not os.path.exists(library_path()):
+        pytest.skip('libbqcjk.so not built')
+    from byteqc_amd.systems import water_cluster
+    mol = M(atom=water_cluster(3, 7.0, seed=5), basis='cc-pvdz')
+    plan = orc.build_plan(mol)
+    ng = len(plan.uniq_l_ctr)
+    cp_i, cp_j = numpy.tril_indices(ng)
+    cp_grp = numpy.ascontiguousarray(numpy.stack([cp_i, cp_j], 1), dtype=numpy.int32)
+    log_q = numpy.ascontiguousarray(numpy.concatenate(plan.log_qs))
+    rng = numpy.random.default_rng(7)
+    ncp = len(plan.log_qs)
+    lib = load_library()
+    for trial in range(6):
+        cond = 10 ** rng.uniform(-16, 0, (ng, ng))
+        cond = numpy.ascontiguousarray(numpy.maximum(cond, cond.T))
+        if trial == 0:
+            cond[:] = 0.0
+        nbins = (10, 3, 17)[trial % 3]
+        ref_cp, ref_bins, ref_sub = orc.task_list(plan, cond, nbins, prescreen)
+        nmax = ncp * (ncp + 1) // 2
+        tcp = numpy.zeros((nmax, 2), dtype=numpy.int32)
+        tb = numpy.zeros((nmax, 2 * (nbins + 1)), dtype=numpy.int32)
+        td = numpy.zeros(nmax)
+        nt = ctypes.c_int(0)
+        assert lib.BQCplan_tasks(ncp, hf._ptr(cp_grp), hf._ptr(plan.bas_pairs_locs),
+                                 hf._ptr(log_q), hf._ptr(cond), ng, plan.cutoff, nbins,
+                                 int(prescreen), hf._ptr(tcp), hf._ptr(tb), hf._ptr(td),
+                                 ctypes.byref(nt)) == 0
+        n = nt.value
+        assert n == len(ref_cp), trial
+        assert numpy.array_equal(tcp[:n], ref_cp)
+        assert numpy.array_equal(tb[:n], ref_bins)
+        assert numpy.array_equal(td[:n], ref_sub)
+        if trial == 0:
+            assert n == (0 if prescreen else nmax)
