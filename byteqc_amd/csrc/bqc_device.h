@@ -47,9 +47,16 @@ __host__ __device__ constexpr int cart_z(int l, int c) { return l - cart_x(l, c)
 // 2*pi^(5/2): prefactor of the Rys integrals (cuobc/lib/gint/g2e.c:41)
 constexpr double TWO_PI_52 = 34.98683665524972497;
 
+// Number of doubles of the interpolation table of N roots.
+constexpr int rys_table_size(int n) { return rys_xbig(n) * 2 * n * (BQC_RYS_DEG + 1); }
+
+// Nodes u_i = t_i^2 / (1 - t_i^2) and weights of the N-point Rys rule for
+// argument x.  tab: the N-root part of the table (global memory, or a copy in
+// LDS made by the caller).
 template <int N>
 __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
-                                          double *__restrict__ w) {
+                                          double *__restrict__ w,
+                                          const double *__restrict__ tab) {
     static_assert(N >= 1 && N <= BQC_RYS_NMAX, "unsupported number of roots");
     if (x >= (double)rys_xbig(N)) {
         const double sq = rsqrt(x);
@@ -64,8 +71,7 @@ __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
     const int it = (int)x;
     const double y = 2.0 * (x - (double)it) - 1.0;
     const double y2 = 2.0 * y;
-    const double *__restrict__ c =
-        BQC_RYS_COEF + rys_offset(N) + it * (2 * N * (BQC_RYS_DEG + 1));
+    const double *__restrict__ c = tab + it * (2 * N * (BQC_RYS_DEG + 1));
 #pragma unroll
     for (int f = 0; f < 2 * N; ++f) {
         const double *cf = c + f * (BQC_RYS_DEG + 1);
@@ -80,6 +86,12 @@ __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
         if (f < N) u[f] = v;
         else w[f - N] = v;
     }
+}
+
+template <int N>
+__device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
+                                          double *__restrict__ w) {
+    rys_roots<N>(x, u, w, BQC_RYS_COEF + rys_offset(N));
 }
 
 // Class traits of a quartet (LI LJ | LK LL).

@@ -277,8 +277,10 @@ static PairData make_pair_data(const BasisProdCache *bp) {
 // ---------------------------------------------------------------------------
 #define BQC_WAVE_CK 32
 
-static int g_ck_rows = 64;   // Q pairs per row-kernel item
-static int g_ck_wave = 32;   // Q pairs per wave-kernel item
+#define BQC_CK_ROWS_DEFAULT 64
+#define BQC_CK_WAVE_DEFAULT 32
+static int g_ck_rows = BQC_CK_ROWS_DEFAULT;  // Q pairs per row-kernel item
+static int g_ck_wave = BQC_CK_WAVE_DEFAULT;  // Q pairs per wave-kernel item
 static std::vector<std::pair<long long, int>> g_ck_override;  // (key, ck)
 
 static int tile_for(int kind, int cls, int prec) {
@@ -709,8 +711,15 @@ void BQCgraph_destroy(void *handle) {
 }
 
 // Q pairs per work item: kind 1 row kernels, 4 wave kernels; cls = class code
-// li*1000+lj*100+lk*10+ll as evaluated, or -1 for the kind's default.
+// li*1000+lj*100+lk*10+ll as evaluated, or -1 for the kind's default; kind 0
+// drops every override and restores the defaults.
 int BQCset_tile(int kind, int cls, int prec, int ck) {
+    if (kind == 0) {
+        g_ck_override.clear();
+        g_ck_rows = BQC_CK_ROWS_DEFAULT;
+        g_ck_wave = BQC_CK_WAVE_DEFAULT;
+        return 0;
+    }
     if (ck < 1 || ck > 4096 || (kind != 1 && kind != 4)) return 1;
     if (cls < 0) {
         if (kind == 1) g_ck_rows = ck;

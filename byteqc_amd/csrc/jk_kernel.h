@@ -90,7 +90,9 @@ template <int LA, int LB, int LC, int LD, typename T, int XN = Cls<LA, LB, LC, L
 __device__ __forceinline__ void eri_block(T *__restrict__ gout, const PairData &pd,
                                           int pp0, int np_p, int qq0, int np_q, double norm,
                                           const double *ra, const double *rb,
-                                          const double *rc, const double *rd, int x0 = 0) {
+                                          const double *rc, const double *rd, int x0 = 0,
+                                          const double *rys_tab = nullptr,
+                                          const double *p_first = nullptr) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NR = C::NR;
     constexpr int NG = XN * C::NFJ * C::NFK * C::NFL;
@@ -100,8 +102,13 @@ __device__ __forceinline__ void eri_block(T *__restrict__ gout, const PairData &
     const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
     for (int ip = 0; ip < np_p; ++ip) {
         const int p = pp0 + ip;
-        const double aij = pd.a12[p], eij = pd.e12[p];
-        const double P[3] = {pd.x12[p], pd.y12[p], pd.z12[p]};
+        // p_first: the first P primitive pair (a12, e12, x12, y12, z12) kept
+        // in registers by the caller
+        const bool pre = p_first != nullptr && ip == 0;
+        const double aij = pre ? p_first[0] : pd.a12[p];
+        const double eij = pre ? p_first[1] : pd.e12[p];
+        const double P[3] = {pre ? p_first[2] : pd.x12[p], pre ? p_first[3] : pd.y12[p],
+                             pre ? p_first[4] : pd.z12[p]};
         for (int iq = 0; iq < np_q; ++iq) {
             const int q = qq0 + iq;
             const double akl = pd.a12[q], ekl = pd.e12[q];
@@ -112,7 +119,8 @@ __device__ __forceinline__ void eri_block(T *__restrict__ gout, const PairData &
             const double a0 = a1 / aijkl;
             const double x = a0 * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]);
             double u[NR], w[NR];
-            rys_roots<NR>(x, u, w);
+            if (rys_tab) rys_roots<NR>(x, u, w, rys_tab);
+            else rys_roots<NR>(x, u, w);
             const double fac = eij * ekl / (sqrt(aijkl) * a1);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {

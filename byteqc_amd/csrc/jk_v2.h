@@ -45,6 +45,17 @@ __global__ void __launch_bounds__(256)
     jk_rows(const TaskDesc *__restrict__ tasks, const int4 *__restrict__ items, int nitems) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NA = C::NFI, NB = C::NFJ, NC = C::NFK, ND = C::NFL;
+    // Rys interpolation table of this class's root count staged in LDS
+    // (8 KB for 1 root, 20 KB for 2) before any wave leaves
+    constexpr bool RYS_LDS = C::NR <= 2;
+    constexpr int RYS_N = RYS_LDS ? rys_table_size(C::NR) : 1;
+    __shared__ double rys_lds[RYS_N];
+    if (RYS_LDS) {
+        const double *src = BQC_RYS_COEF + rys_offset(C::NR);
+        for (int i = threadIdx.x; i < RYS_N; i += blockDim.x) rys_lds[i] = src[i];
+        __syncthreads();
+    }
+    const double *rys_tab = RYS_LDS ? rys_lds : nullptr;
     const int lane = threadIdx.x & 63;
     const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (item >= nitems) return;
@@ -79,9 +90,12 @@ __global__ void __launch_bounds__(256)
     const int a0 = pvalid ? pd.ao_loc[sa] : 0;
     const int b0 = pvalid ? pd.ao_loc[sb] : 0;
     double ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+    double p_first[5] = {1, 0, 0, 0, 0};
     if (pvalid) {
         ra[0] = pd.bas_x[sa]; ra[1] = pd.bas_y[sa]; ra[2] = pd.bas_z[sa];
         rb[0] = pd.bas_x[sb]; rb[1] = pd.bas_y[sb]; rb[2] = pd.bas_z[sb];
+        p_first[0] = pd.a12[pp0]; p_first[1] = pd.e12[pp0];
+        p_first[2] = pd.x12[pp0]; p_first[3] = pd.y12[pp0]; p_first[4] = pd.z12[pp0];
     }
     // runs of equal first shell a among the 64 lanes
     const int key_next = __shfl_down(sa, 1, 64);
@@ -111,6 +125,9 @@ __global__ void __launch_bounds__(256)
 
         int gq = tk.bas_q0 + tk.perm_q[q0];
         int sc = __builtin_amdgcn_readfirstlane(pd.bra[gq]);
+        // D(b,c), D(a,c): loaded at the start of every run of equal c
+        double dbc[NB * NC], dac[NA * NC];
+        bool new_c = true;
         for (int q = q0; q < q1; ++q) {
             const int qpos = __builtin_amdgcn_readfirstlane(tk.perm_q[q]);
             gq = tk.bas_q0 + qpos;
@@ -127,7 +144,8 @@ __global__ void __launch_bounds__(256)
                 const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
                 const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
                 eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p, tk.prim_q0 + qpos * tk.nprim_q,
-                                          tk.nprim_q, norm, ra, rb, rc, rd);
+                                          tk.nprim_q, norm, ra, rb, rc, rd, 0, rys_tab,
+                                          p_first);
             } else {
 #pragma unroll
                 for (int n = 0; n < C::NF; ++n) g[n] = (T)0;
@@ -161,7 +179,7 @@ __global__ void __launch_bounds__(256)
             }
             if (VK) {
                 // K(a,c) += g D(b,d); K(b,c) += g D(a,d); K(a,d) += g D(b,c); K(b,d) += g D(a,c)
-                double dbd[NB * ND], dad[NA * ND], dbc[NB * NC], dac[NA * NC];
+                double dbd[NB * ND], dad[NA * ND];
 #pragma unroll
                 for (int w = 0; w < ND; ++w) {
 #pragma unroll
@@ -171,14 +189,16 @@ __global__ void __launch_bounds__(256)
                     for (int x = 0; x < NA; ++x)
                         dad[x * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + a0 + x] : 0.0;
                 }
+                if (new_c) {
 #pragma unroll
-                for (int z = 0; z < NC; ++z) {
+                    for (int z = 0; z < NC; ++z) {
 #pragma unroll
-                    for (int y = 0; y < NB; ++y)
-                        dbc[y * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + b0 + y] : 0.0;
+                        for (int y = 0; y < NB; ++y)
+                            dbc[y * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + b0 + y] : 0.0;
 #pragma unroll
-                    for (int x = 0; x < NA; ++x)
-                        dac[x * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + a0 + x] : 0.0;
+                        for (int x = 0; x < NA; ++x)
+                            dac[x * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + a0 + x] : 0.0;
+                    }
                 }
 #pragma unroll
                 for (int x = 0; x < NA; ++x)
@@ -243,6 +263,7 @@ __global__ void __launch_bounds__(256)
                         kbc[y * NC + z] = 0.0;
                     }
             }
+            new_c = sc_next != sc;
             sc = sc_next;
         }
         if (VJ && pvalid) {

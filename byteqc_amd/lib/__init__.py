@@ -40,7 +40,7 @@ _SIGNATURES = {
     'BQCset_tile': (c_int, [c_int, c_int, c_int, c_int]),
     'BQCplan_tasks': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double,
                               c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
-    'BQCdm_cond': (c_int, [c_void_p] * 7 + [c_int, c_int, c_void_p]),
+    'BQCdm_cond': (c_int, [c_void_p] * 7 + [c_int, c_int, c_int, c_void_p]),
     'BQCsymmetrize_jk': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
     'BQClast_build_flops': (None, [c_void_p]),
     'BQCprofile_enable': (None, [c_int]),

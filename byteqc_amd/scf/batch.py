@@ -16,7 +16,6 @@ from __future__ import annotations
 
 import ctypes
 
-import numpy
 import torch
 
 from byteqc_amd.lib import load_library
@@ -25,17 +24,22 @@ from byteqc_amd.scf import hf
 _PREC = {'fp64': 0, 'mixed': 1}
 
 
-def _parray(ptrs):
-    return (ctypes.c_void_p * len(ptrs))(*[p if p is None else ctypes.c_void_p(p) for p in ptrs])
-
-
 class BatchJK:
-    """J and K for a list of molecules (each with its own plan)."""
+    """J and K for a list of molecules (each with its own plan).
+
+    mols: the molecules (plans are built for them when vhfopts is None), or
+    None when vhfopts (built plans, one per molecule) are given."""
 
     def __init__(self, mols, vhfopts=None, precision='fp64', fp32_cut=1e-7, nbins=None,
                  with_j=True, with_k=True, stream=None):
-        self.mols = list(mols)
-        self.opts = vhfopts or [hf._VHFOpt(m).build() for m in self.mols]
+        if vhfopts is None:
+            self.mols = list(mols)
+            self.opts = [hf._VHFOpt(m).build() for m in self.mols]
+        else:
+            self.opts = list(vhfopts)
+            self.mols = list(mols) if mols is not None else [o.mol for o in self.opts]
+        if not self.opts:
+            raise ValueError('empty batch')
         self.precision = precision
         self.fp32_cut = fp32_cut
         self.with_j, self.with_k = with_j, with_k
@@ -107,6 +111,10 @@ class BatchJK:
     # -- public ------------------------------------------------------------------
     def build(self, dms):
         """One batched J/K build; returns [(J, K)] device tensors per molecule."""
+        with hf.on_stream(self.stream):
+            return self._build(dms)
+
+    def _build(self, dms):
         self._load(dms)
         self._zero()
         keep, args = self._args(self._plan())
@@ -118,6 +126,10 @@ class BatchJK:
 
     def capture(self, dms):
         """Record the kernels of the build for dms into a HIP graph."""
+        with hf.on_stream(self.stream):
+            return self._capture(dms)
+
+    def _capture(self, dms):
         self.release()
         self._load(dms)
         keep, args = self._args(self._plan())
@@ -131,17 +143,18 @@ class BatchJK:
 
     def replay(self):
         """Zero J/K, replay the captured kernels and return [(J, K)]."""
-        self._zero()
-        err = load_library().BQCgraph_launch(self._graph, hf._stream_ptr(self.stream))
-        if err:
-            raise RuntimeError('graph launch failed')
-        return self._finish()
+        with hf.on_stream(self.stream):
+            self.replay_kernels()
+            return self._finish()
 
     def replay_kernels(self):
         """Zero J/K and replay the captured kernels only (no transforms)."""
-        self._zero()
-        if load_library().BQCgraph_launch(self._graph, hf._stream_ptr(self.stream)):
-            raise RuntimeError('graph launch failed')
+        if not self._graph:
+            raise RuntimeError('no captured graph')
+        with hf.on_stream(self.stream):
+            self._zero()
+            if load_library().BQCgraph_launch(self._graph, hf._stream_ptr(self.stream)):
+                raise RuntimeError('graph launch failed')
 
     def release(self):
         if self._graph:

@@ -25,6 +25,7 @@ Differences from the reference, all deliberate:
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import time
 
@@ -79,6 +80,12 @@ def _ptr(x):
 def _stream_ptr(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+def on_stream(stream):
+    """Context that makes ``stream`` torch's current stream (no-op for None),
+    so torch ops and library launches of one call share a stream."""
+    return torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
 
 
 # ---------------------------------------------------------------------------
@@ -401,7 +408,7 @@ def dm_cond_batch(dmcs, locs, stream=None):
     out = torch.zeros(int((ng * ng).sum()), dtype=torch.float64, device=dev)
     err = load_library().BQCdm_cond(_ptr(ptrs), _ptr(nao), _ptr(loc_d), _ptr(off_d), _ptr(ng_d),
                                     _ptr(oo_d), _ptr(out), len(dmcs), int(ng.max()),
-                                    _stream_ptr(stream))
+                                    max(int(m.shape[-1]) for m in dmcs), _stream_ptr(stream))
     if err:
         raise RuntimeError('density bound kernel failed')
     host = out.cpu().numpy()
@@ -435,6 +442,10 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None,
     """
     if hermi != 1:
         raise NotImplementedError('JK-builder only supports hermitian density matrix')
+    if stream is not None:
+        with on_stream(stream):
+            return get_jk(mol, dm, hermi, vhfopt, with_j, with_k, omega, verbose, group_size,
+                          gpus, prescreen, nbins, precision, fp32_cut, return_device, None)
     if vhfopt is None:
         vhfopt = _VHFOpt(mol, 'int2e').build(group_size=group_size)
     dev = _device()

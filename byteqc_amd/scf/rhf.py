@@ -70,6 +70,10 @@ class RHF:
 
     # -- set-up --------------------------------------------------------------
     def build(self):
+        with hf.on_stream(self.stream):
+            return self._build()
+
+    def _build(self):
         t0 = time.perf_counter()
         if self._opt is None:
             self._opt = hf._VHFOpt(self.mol).build()
@@ -109,9 +113,23 @@ class RHF:
     # -- one iteration ---------------------------------------------------------
     def step(self):
         """One SCF iteration; returns True once converged."""
-        prec = self._jk_precision()
+        with hf.on_stream(self.stream):
+            return self._step()
+
+    def _step(self):
+        prec = self.next_precision()
         j, k = hf.get_jk(self.mol, self.dm, vhfopt=self._opt, return_device=True,
                          precision=prec, fp32_cut=self.fp32_cut, stream=self.stream)
+        return self.absorb(j, k, prec)
+
+    def next_precision(self):
+        """Precision of the next J/K build ('fp64' or 'mixed')."""
+        return self._jk_precision()
+
+    def absorb(self, j, k, prec):
+        """Advance one iteration with J, K of the current density (built at
+        precision prec by any caller, e.g. a batched build); returns True
+        once converged."""
         self.jk_builds[prec] += 1
         f = self.H + j - 0.5 * k
         e_elec = 0.5 * float(torch.sum(self.dm * (self.H + f)))

@@ -142,8 +142,11 @@ void BQCgraph_destroy(void *graph);
 
 /* Q pairs per work item of the row kernels (kind 1) or wave kernels
  * (kind 4), for class code cls (li*1000+lj*100+lk*10+ll as evaluated) and
- * precision prec (0 FP64, 1 FP32), or for all classes of the kind (cls < 0).
- * Used by the tile autotuner.  Returns 0 ok, 1 bad argument. */
+ * precision prec (0 FP64, 1 FP32), or for all classes of the kind (cls < 0);
+ * kind 0 clears all per-class settings and restores the defaults (64, 32).
+ * Process-wide setting read when a build is planned; used by the tile
+ * autotuner.  No counterpart in the reference (its launch shapes are fixed
+ * per class, nr_jk_driver.cu:46-225).  Returns 0 ok, 1 bad argument. */
 int BQCset_tile(int kind, int cls, int prec, int ck);
 
 /* Task list of one build on the host (the per-task loop of
@@ -160,9 +163,11 @@ int BQCplan_tasks(int ncp, const int *cp_grp, const int *bas_pairs_locs, const d
  * blocks.  mats (device array of device pointers), nao, off, ng, outoff:
  * device int arrays per matrix; loc (device): AO offsets of the groups of
  * matrix m at loc[off[m] .. off[m]+ng[m]]; out (device) receives ng[m]^2
- * values per matrix at outoff[m]; gmax = max ng (<= 64). */
+ * values per matrix at outoff[m] and must be zeroed by the caller;
+ * gmax = max ng (<= 256), naomax = max nao.  Replaces the host-side
+ * lib.condense('absmax', ...) of cuobc/scf/hf.py:106-107. */
 int BQCdm_cond(const double *const *mats, const int *nao, const int *loc, const int *off,
-               const int *ng, const int *outoff, double *out, int nmat, int gmax,
+               const int *ng, const int *outoff, double *out, int nmat, int gmax, int naomax,
                hipStream_t stream);
 
 /* Schwarz bounds on the GPU: q[i*nbas + j] = sqrt(max |(ij|ij)|) over the

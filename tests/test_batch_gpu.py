@@ -89,3 +89,22 @@ def test_batch_mixed_precision_energy():
         eb = 0.5 * numpy.sum(d * (b[0].cpu().numpy() - 0.5 * b[1].cpu().numpy()))
         assert abs(ea - eb) < 1e-8
         assert _maxdiff(a[0].cpu(), b[0].cpu()) < 1e-8
+
+
+def test_autotuned_tiles_keep_results():
+    """Any tile choice gives the same J/K (only the accumulation order moves)."""
+    _torch_gpu()
+    from byteqc_amd.scf import autotune
+    from byteqc_amd.scf.batch import BatchJK
+    mols, dms = _batch_inputs()
+    bj = BatchJK(mols)
+    try:
+        base = bj.build(dms)
+        table = autotune.tune(lambda: bj.build(dms), candidates={1: (16, 256), 4: (8, 128)})
+        assert table and all(rec['ck'] in rec['ms'] for rec in table.values())
+        got = bj.build(dms)
+        for a, b in zip(base, got):
+            assert _maxdiff(a[0].cpu(), b[0].cpu()) < 1e-11
+            assert _maxdiff(a[1].cpu(), b[1].cpu()) < 1e-11
+    finally:
+        autotune.reset()

@@ -191,3 +191,20 @@ def test_task_planner_matches_oracle(prescreen):
         assert numpy.array_equal(td[:n], ref_sub)
         if trial == 0:
             assert n == (0 if prescreen else nmax)
+
+
+def test_poisson_workload_and_footprint_estimate():
+    from byteqc_amd.scf.serve import poisson_workload, estimate_footprint
+    from byteqc_amd.systems import water_mole, box_for_density
+    a = poisson_workload(2.0, 200.0, seed=4)
+    assert a == poisson_workload(2.0, 200.0, seed=4)
+    t = numpy.array([x[0] for x in a])
+    n = numpy.array([x[1] for x in a])
+    assert numpy.all(numpy.diff(t) > 0) and t[-1] < 200.0
+    assert abs(len(a) / 200.0 - 2.0) < 0.3             # Poisson rate
+    assert n.min() >= 5 and n.max() <= 120
+    assert abs(numpy.median(numpy.log(n)) - numpy.log(numpy.sqrt(5 * 120))) < 0.3
+    fp = [estimate_footprint(water_mole(k, box_for_density(k), 1, 'def2-svp'))
+          for k in (1, 2, 4, 8)]
+    assert all(b > a_ for a_, b in zip(fp, fp[1:]))
+    assert fp[3] > 3 * fp[1]
