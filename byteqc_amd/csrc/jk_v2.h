@@ -22,20 +22,57 @@
 
 #define BQC_V2_CK 64
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+// Cross-lane moves of a double through DPP (two 32-bit halves).  Lanes
+// whose source is outside the pattern, or whose row is masked off, read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// DPP controls (GFX9 encoding)
+#define BQC_DPP_ROW_SHR(k) (0x110 + (k))
+#define BQC_DPP_ROW_BCAST15 0x142
+#define BQC_DPP_ROW_BCAST31 0x143
+
+// Inclusive prefix sum of v over the lanes of the run containing this lane
+// (runs are maximal blocks of consecutive lanes; run_start is the first lane
+// of this lane's run): Hillis-Steele within rows of 16 lanes, then the row
+// broadcasts of lane 15 and lane 31.  The last lane of a run holds the run
+// total.  With run_start = 0 everywhere, lane 63 holds the wave total.
+__device__ __forceinline__ double run_scan(double v, int lane, int run_start) {
+    double t = dpp_d<BQC_DPP_ROW_SHR(1)>(v);
+    if (lane - 1 >= run_start) v += t;
+    t = dpp_d<BQC_DPP_ROW_SHR(2)>(v);
+    if (lane - 2 >= run_start) v += t;
+    t = dpp_d<BQC_DPP_ROW_SHR(4)>(v);
+    if (lane - 4 >= run_start) v += t;
+    t = dpp_d<BQC_DPP_ROW_SHR(8)>(v);
+    if (lane - 8 >= run_start) v += t;
+    // rows 1 and 3 receive lane 15 and lane 47
+    t = dpp_d<BQC_DPP_ROW_BCAST15, 0xa>(v);
+    if ((lane & 16) && run_start <= (lane | 15) - 16) v += t;
+    // rows 2 and 3 receive lane 31
+    t = dpp_d<BQC_DPP_ROW_BCAST31, 0xc>(v);
+    if (lane >= 32 && run_start <= 31) v += t;
     return v;
 }
 
-// sum of v over lanes [lane, run_end) delivered to every lane; the run head
-// (first lane of its run) holds the full run sum
-__device__ __forceinline__ double run_sum(double v, int lane, int run_end) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double o = __shfl_down(v, d, 64);
-        if (lane + d < run_end) v += o;
-    }
+// Sum of v over the wavefront, valid in lane 63.
+__device__ __forceinline__ double wave_sum63(double v) {
+    double t = dpp_d<BQC_DPP_ROW_SHR(1)>(v);
+    v += t;
+    t = dpp_d<BQC_DPP_ROW_SHR(2)>(v);
+    v += t;
+    t = dpp_d<BQC_DPP_ROW_SHR(4)>(v);
+    v += t;
+    t = dpp_d<BQC_DPP_ROW_SHR(8)>(v);
+    v += t;
+    t = dpp_d<BQC_DPP_ROW_BCAST15, 0xa>(v);
+    v += t;
+    t = dpp_d<BQC_DPP_ROW_BCAST31, 0xc>(v);
+    v += t;
     return v;
 }
 
@@ -99,10 +136,10 @@ __global__ void __launch_bounds__(256)
     }
     // runs of equal first shell a among the 64 lanes
     const int key_next = __shfl_down(sa, 1, 64);
-    unsigned long long ends = __ballot(lane == 63 || key_next != sa);
-    const int run_end = lane + 1 + (int)__builtin_ctzll(ends >> lane);
+    const bool tail = lane == 63 || key_next != sa;
     const int key_prev = __shfl_up(sa, 1, 64);
-    const bool head = lane == 0 || key_prev != sa;
+    const unsigned long long heads = __ballot(lane == 0 || key_prev != sa);
+    const int run_start = 63 - __builtin_clzll(heads & (~0ull >> (63 - lane)));
 
     const int nao = jk.nao;
     const size_t nao2 = (size_t)nao * nao;
@@ -173,8 +210,8 @@ __global__ void __launch_bounds__(256)
                         for (int x = 0; x < NA; ++x)
 #pragma unroll
                             for (int y = 0; y < NB; ++y) s += G(x, y, z, w) * dab[x * NB + y];
-                        s = wave_sum(s);
-                        if (lane == 0) atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, s);
+                        s = wave_sum63(s);
+                        if (lane == 63) atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, s);
                     }
             }
             if (VK) {
@@ -222,8 +259,8 @@ __global__ void __launch_bounds__(256)
                         for (int y = 0; y < NB; ++y)
 #pragma unroll
                             for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dbc[y * NC + z];
-                        s = run_sum(s, lane, run_end);
-                        if (head && pvalid) atomic_add(VK + (size_t)(d0 + w) * nao + a0 + x, s);
+                        s = run_scan(s, lane, run_start);
+                        if (tail && pvalid) atomic_add(VK + (size_t)(d0 + w) * nao + a0 + x, s);
                     }
                 // K(b,d): per lane
 #pragma unroll
@@ -250,8 +287,8 @@ __global__ void __launch_bounds__(256)
                 for (int x = 0; x < NA; ++x)
 #pragma unroll
                     for (int z = 0; z < NC; ++z) {
-                        const double s = run_sum(kac[x * NC + z], lane, run_end);
-                        if (head && pvalid) atomic_add(VK + (size_t)(c0 + z) * nao + a0 + x, s);
+                        const double s = run_scan(kac[x * NC + z], lane, run_start);
+                        if (tail && pvalid) atomic_add(VK + (size_t)(c0 + z) * nao + a0 + x, s);
                         kac[x * NC + z] = 0.0;
                     }
 #pragma unroll
