@@ -11,6 +11,7 @@
 //    compile-time bounds so the intermediates live in registers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #define BQC_RYS_QUAL static __device__
 #include "rys_table.inc"
@@ -51,12 +52,11 @@ constexpr double TWO_PI_52 = 34.98683665524972497;
 constexpr int rys_table_size(int n) { return rys_xbig(n) * 2 * n * (BQC_RYS_DEG + 1); }
 
 // Nodes u_i = t_i^2 / (1 - t_i^2) and weights of the N-point Rys rule for
-// argument x.  tab: the N-root part of the table (global memory, or a copy in
-// LDS made by the caller).
-template <int N>
+// argument x.  tab: the N-root part of the table (a pointer into global
+// memory, or into a copy in LDS made by the caller).
+template <int N, typename TabPtr>
 __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
-                                          double *__restrict__ w,
-                                          const double *__restrict__ tab) {
+                                          double *__restrict__ w, TabPtr tab) {
     static_assert(N >= 1 && N <= BQC_RYS_NMAX, "unsupported number of roots");
     if (x >= (double)rys_xbig(N)) {
         const double sq = rsqrt(x);
@@ -71,10 +71,10 @@ __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
     const int it = (int)x;
     const double y = 2.0 * (x - (double)it) - 1.0;
     const double y2 = 2.0 * y;
-    const double *__restrict__ c = tab + it * (2 * N * (BQC_RYS_DEG + 1));
+    const auto c = tab + it * (2 * N * (BQC_RYS_DEG + 1));
 #pragma unroll
     for (int f = 0; f < 2 * N; ++f) {
-        const double *cf = c + f * (BQC_RYS_DEG + 1);
+        const auto cf = c + f * (BQC_RYS_DEG + 1);
         double b1 = 0.0, b2 = 0.0;
 #pragma unroll
         for (int j = BQC_RYS_DEG; j >= 1; --j) {
@@ -91,7 +91,7 @@ __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
 template <int N>
 __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
                                           double *__restrict__ w) {
-    rys_roots<N>(x, u, w, BQC_RYS_COEF + rys_offset(N));
+    rys_roots<N>(x, u, w, (const double *)(BQC_RYS_COEF + rys_offset(N)));
 }
 
 // Class traits of a quartet (LI LJ | LK LL).

@@ -81,17 +81,36 @@ __device__ __forceinline__ void atomic_add(double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Address-space views of device buffers.  Plan data, the density and the
+// pair lists are not written while a J/K kernel runs: reads with a
+// wave-uniform index go through the constant space (scalar loads), per-lane
+// reads through the global space (global_load rather than flat loads, which
+// would also hold up LDS and scalar waits).  J/K accumulators are updated
+// with global atomics.
+template <typename T> using cptr = const __attribute__((address_space(4))) T *;
+template <typename T> using gptr = const __attribute__((address_space(1))) T *;
+typedef __attribute__((address_space(1))) double gdouble;
+template <typename T> __device__ __forceinline__ cptr<T> cst(const T *p) { return (cptr<T>)p; }
+template <typename T> __device__ __forceinline__ gptr<T> glb(const T *p) { return (gptr<T>)p; }
+__device__ __forceinline__ gdouble *glb_rw(double *p) { return (gdouble *)p; }
+__device__ __forceinline__ void atomic_add(gdouble *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Integral block of (p | q): p = shells (a, b) of the "bra" pair, q = (c, d).
 // T is the arithmetic type of the recurrence and of the contracted block;
 // roots, weights, prefactors and recurrence coefficients are always FP64.
 // XN consecutive Cartesian components of shell a starting at x0 are
 // evaluated (XN = NFI: the whole block); gout holds XN*NFJ*NFK*NFL values.
-template <int LA, int LB, int LC, int LD, typename T, int XN = Cls<LA, LB, LC, LD>::NFI>
+// rys_tab: nullptr (table in global memory) or a pointer to the table of
+// this class's root count in LDS.
+template <int LA, int LB, int LC, int LD, typename T, int XN = Cls<LA, LB, LC, LD>::NFI,
+          typename RysPtr = std::nullptr_t>
 __device__ __forceinline__ void eri_block(T *__restrict__ gout, const PairData &pd,
                                           int pp0, int np_p, int qq0, int np_q, double norm,
                                           const double *ra, const double *rb,
                                           const double *rc, const double *rd, int x0 = 0,
-                                          const double *rys_tab = nullptr,
+                                          RysPtr rys_tab = nullptr,
                                           const double *p_first = nullptr) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NR = C::NR;
@@ -100,27 +119,33 @@ __device__ __forceinline__ void eri_block(T *__restrict__ gout, const PairData &
     for (int n = 0; n < NG; ++n) gout[n] = (T)0;
     const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
     const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
+    // P primitives: per-lane reads; Q primitives: the same for all lanes of a
+    // row-kernel wavefront (scalar loads there)
+    const gptr<double> pa12 = glb(pd.a12), pe12 = glb(pd.e12), px12 = glb(pd.x12),
+                       py12 = glb(pd.y12), pz12 = glb(pd.z12);
+    const cptr<double> qa12 = cst(pd.a12), qe12 = cst(pd.e12), qx12 = cst(pd.x12),
+                       qy12 = cst(pd.y12), qz12 = cst(pd.z12);
     for (int ip = 0; ip < np_p; ++ip) {
         const int p = pp0 + ip;
         // p_first: the first P primitive pair (a12, e12, x12, y12, z12) kept
         // in registers by the caller
         const bool pre = p_first != nullptr && ip == 0;
-        const double aij = pre ? p_first[0] : pd.a12[p];
-        const double eij = pre ? p_first[1] : pd.e12[p];
-        const double P[3] = {pre ? p_first[2] : pd.x12[p], pre ? p_first[3] : pd.y12[p],
-                             pre ? p_first[4] : pd.z12[p]};
+        const double aij = pre ? p_first[0] : pa12[p];
+        const double eij = pre ? p_first[1] : pe12[p];
+        const double P[3] = {pre ? p_first[2] : px12[p], pre ? p_first[3] : py12[p],
+                             pre ? p_first[4] : pz12[p]};
         for (int iq = 0; iq < np_q; ++iq) {
             const int q = qq0 + iq;
-            const double akl = pd.a12[q], ekl = pd.e12[q];
-            const double Q[3] = {pd.x12[q], pd.y12[q], pd.z12[q]};
+            const double akl = qa12[q], ekl = qe12[q];
+            const double Q[3] = {qx12[q], qy12[q], qz12[q]};
             const double pq[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
             const double aijkl = aij + akl;
             const double a1 = aij * akl;
             const double a0 = a1 / aijkl;
             const double x = a0 * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]);
             double u[NR], w[NR];
-            if (rys_tab) rys_roots<NR>(x, u, w, rys_tab);
-            else rys_roots<NR>(x, u, w);
+            if constexpr (std::is_same<RysPtr, std::nullptr_t>::value) rys_roots<NR>(x, u, w);
+            else rys_roots<NR>(x, u, w, rys_tab);
             const double fac = eij * ekl / (sqrt(aijkl) * a1);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {

@@ -92,7 +92,8 @@ __global__ void __launch_bounds__(256)
         for (int i = threadIdx.x; i < RYS_N; i += blockDim.x) rys_lds[i] = src[i];
         __syncthreads();
     }
-    const double *rys_tab = RYS_LDS ? rys_lds : nullptr;
+    typedef const __attribute__((address_space(3))) double *lds_ptr;
+    const lds_ptr rys_tab = (lds_ptr)rys_lds;
     const int lane = threadIdx.x & 63;
     const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (item >= nitems) return;
@@ -111,7 +112,7 @@ __global__ void __launch_bounds__(256)
         int s = 0;
         while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
         qlim = tk.qbins[tk.nbins - s];
-        ppos = tk.perm_p[pidx];
+        ppos = glb(tk.perm_p)[pidx];
     }
     const int qend_item = min(qe, tk.qbins[tk.nbins]);
     int qlim_max = qlim;
@@ -120,19 +121,26 @@ __global__ void __launch_bounds__(256)
     const int q1 = min(qend_item, __builtin_amdgcn_readfirstlane(qlim_max));
     if (q1 <= q0) return;
 
+    // per-lane (P side) and wave-uniform (Q side) views of the plan data
+    const gptr<int> lbra = glb(pd.bra), lket = glb(pd.ket), lao = glb(pd.ao_loc);
+    const gptr<double> lbx = glb(pd.bas_x), lby = glb(pd.bas_y), lbz = glb(pd.bas_z);
+    const cptr<int> ubra = cst(pd.bra), uket = cst(pd.ket), uao = cst(pd.ao_loc);
+    const cptr<int> uperm_q = cst(tk.perm_q);
+    const cptr<double> ubx = cst(pd.bas_x), uby = cst(pd.bas_y), ubz = cst(pd.bas_z);
     const int gp = tk.bas_p0 + ppos;
-    const int sa = pvalid ? pd.bra[gp] : -1;
-    const int sb = pvalid ? pd.ket[gp] : 0;
+    const int sa = pvalid ? lbra[gp] : -1;
+    const int sb = pvalid ? lket[gp] : 0;
     const int pp0 = tk.prim_p0 + ppos * tk.nprim_p;
-    const int a0 = pvalid ? pd.ao_loc[sa] : 0;
-    const int b0 = pvalid ? pd.ao_loc[sb] : 0;
+    const int a0 = pvalid ? lao[sa] : 0;
+    const int b0 = pvalid ? lao[sb] : 0;
     double ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
     double p_first[5] = {1, 0, 0, 0, 0};
     if (pvalid) {
-        ra[0] = pd.bas_x[sa]; ra[1] = pd.bas_y[sa]; ra[2] = pd.bas_z[sa];
-        rb[0] = pd.bas_x[sb]; rb[1] = pd.bas_y[sb]; rb[2] = pd.bas_z[sb];
-        p_first[0] = pd.a12[pp0]; p_first[1] = pd.e12[pp0];
-        p_first[2] = pd.x12[pp0]; p_first[3] = pd.y12[pp0]; p_first[4] = pd.z12[pp0];
+        ra[0] = lbx[sa]; ra[1] = lby[sa]; ra[2] = lbz[sa];
+        rb[0] = lbx[sb]; rb[1] = lby[sb]; rb[2] = lbz[sb];
+        p_first[0] = glb(pd.a12)[pp0]; p_first[1] = glb(pd.e12)[pp0];
+        p_first[2] = glb(pd.x12)[pp0]; p_first[3] = glb(pd.y12)[pp0];
+        p_first[4] = glb(pd.z12)[pp0];
     }
     // runs of equal first shell a among the 64 lanes
     const int key_next = __shfl_down(sa, 1, 64);
@@ -144,9 +152,10 @@ __global__ void __launch_bounds__(256)
     const int nao = jk.nao;
     const size_t nao2 = (size_t)nao * nao;
     for (int idm = 0; idm < jk.n_dm; ++idm) {
-        const double *__restrict__ D = jk.dm + idm * nao2;
-        double *VJ = jk.vj ? jk.vj + idm * nao2 : nullptr;
-        double *VK = jk.vk ? jk.vk + idm * nao2 : nullptr;
+        const gptr<double> D = glb(jk.dm + idm * nao2);      // per-lane reads
+        const cptr<double> DU = cst(jk.dm + idm * nao2);     // wave-uniform reads
+        gdouble *VJ = jk.vj ? glb_rw(jk.vj + idm * nao2) : nullptr;
+        gdouble *VK = jk.vk ? glb_rw(jk.vk + idm * nao2) : nullptr;
         double dab[NA * NB];
 #pragma unroll
         for (int x = 0; x < NA; ++x)
@@ -160,16 +169,16 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
 
-        int gq = tk.bas_q0 + tk.perm_q[q0];
-        int sc = __builtin_amdgcn_readfirstlane(pd.bra[gq]);
+        int gq = tk.bas_q0 + uperm_q[q0];
+        int sc = __builtin_amdgcn_readfirstlane(ubra[gq]);
         // D(b,c), D(a,c): loaded at the start of every run of equal c
         double dbc[NB * NC], dac[NA * NC];
         bool new_c = true;
         for (int q = q0; q < q1; ++q) {
-            const int qpos = __builtin_amdgcn_readfirstlane(tk.perm_q[q]);
+            const int qpos = __builtin_amdgcn_readfirstlane(uperm_q[q]);
             gq = tk.bas_q0 + qpos;
-            const int sd = __builtin_amdgcn_readfirstlane(pd.ket[gq]);
-            const int c0 = pd.ao_loc[sc], d0 = pd.ao_loc[sd];
+            const int sd = __builtin_amdgcn_readfirstlane(uket[gq]);
+            const int c0 = uao[sc], d0 = uao[sd];
             bool active = pvalid && q < qlim;
             double norm = TWO_PI_52;
             if (tk.diag) {
@@ -178,11 +187,16 @@ __global__ void __launch_bounds__(256)
             }
             T g[C::NF];
             if (active) {
-                const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
-                const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
-                eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p, tk.prim_q0 + qpos * tk.nprim_q,
-                                          tk.nprim_q, norm, ra, rb, rc, rd, 0, rys_tab,
-                                          p_first);
+                const double rc[3] = {ubx[sc], uby[sc], ubz[sc]};
+                const double rd[3] = {ubx[sd], uby[sd], ubz[sd]};
+                if constexpr (RYS_LDS)
+                    eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
+                                                 tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
+                                                 norm, ra, rb, rc, rd, 0, rys_tab, p_first);
+                else
+                    eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
+                                                 tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
+                                                 norm, ra, rb, rc, rd, 0, nullptr, p_first);
             } else {
 #pragma unroll
                 for (int n = 0; n < C::NF; ++n) g[n] = (T)0;
@@ -194,7 +208,7 @@ __global__ void __launch_bounds__(256)
                 for (int z = 0; z < NC; ++z)
 #pragma unroll
                     for (int w = 0; w < ND; ++w) {
-                        const double dcd = D[(size_t)(c0 + z) * nao + d0 + w];
+                        const double dcd = DU[(size_t)(c0 + z) * nao + d0 + w];
 #pragma unroll
                         for (int x = 0; x < NA; ++x)
 #pragma unroll
@@ -280,7 +294,7 @@ __global__ void __launch_bounds__(256)
             int sc_next = -1;
             if (q + 1 < q1) {
                 sc_next = __builtin_amdgcn_readfirstlane(
-                    pd.bra[tk.bas_q0 + tk.perm_q[q + 1]]);
+                    ubra[tk.bas_q0 + uperm_q[q + 1]]);
             }
             if (VK && sc_next != sc) {
 #pragma unroll
