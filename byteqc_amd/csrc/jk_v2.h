@@ -76,6 +76,50 @@ __device__ __forceinline__ double wave_sum63(double v) {
     return v;
 }
 
+// Ordering point for LDS traffic between the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wavefront-packed atomics.  Every lane holds NV values for elements of V at
+// offsets off[v] (-1: nothing to add), the components of one lane adjacent
+// in memory.  Staged through LDS, atomic instruction j then covers entries
+// 64j .. 64j+63 in lane-major order, so the components of a lane fall into
+// one instruction and mostly into one 64-byte segment: the atomic units take
+// one request per segment and instruction, so this needs about NV times
+// fewer requests than one instruction per component.
+template <int NV>
+__device__ __forceinline__ void packed_atomic_add(gdouble *V, const double (&val)[NV],
+                                                  const int (&off)[NV], double *sv, int *so,
+                                                  int lane) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        sv[lane * NV + v] = val[v];
+        so[lane * NV + v] = off[v];
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const int e = j * 64 + lane;
+        const int o = so[e];
+        const double x = sv[e];
+        if (o >= 0) atomic_add(V + o, x);
+    }
+    wave_sync();
+}
+
+// Largest packed group of a row-kernel class (groups above 9 components,
+// and single components, are issued directly).
+constexpr int pack_nv(int n) { return n >= 2 && n <= 9 ? n : 0; }
+constexpr int max2(int a, int b) { return a > b ? a : b; }
+template <int NA, int NB, int NC, int ND>
+constexpr int row_pack_max() {
+    return max2(max2(pack_nv(NB * ND), pack_nv(NA * ND)),
+                max2(pack_nv(NA * NC), pack_nv(NB * NC)));
+}
+
 // Work item: (task, chunk of 64 P pairs, first and end Q position).
 template <int LA, int LB, int LC, int LD, typename T>
 __global__ void __launch_bounds__(256)
@@ -94,6 +138,12 @@ __global__ void __launch_bounds__(256)
     }
     typedef const __attribute__((address_space(3))) double *lds_ptr;
     const lds_ptr rys_tab = (lds_ptr)rys_lds;
+    // per-wavefront staging of packed atomics
+    constexpr int PK = row_pack_max<NA, NB, NC, ND>();
+    __shared__ double pk_val[4 * 64 * (PK > 0 ? PK : 1)];
+    __shared__ int pk_off[4 * 64 * (PK > 0 ? PK : 1)];
+    double *const sv = pk_val + (threadIdx.x >> 6) * 64 * (PK > 0 ? PK : 1);
+    int *const so = pk_off + (threadIdx.x >> 6) * 64 * (PK > 0 ? PK : 1);
     const int lane = threadIdx.x & 63;
     const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (item >= nitems) return;
@@ -263,31 +313,56 @@ __global__ void __launch_bounds__(256)
                                 kac[x * NC + z] += v * dbd[y * ND + w];
                                 kbc[y * NC + z] += v * dad[x * ND + w];
                             }
-                // K(a,d): segmented over runs of equal a
+                // K(a,d): segmented over runs of equal a, added by the run tails
+                {
+                    constexpr int NV = NA * ND;
+                    double val[NV];
+                    int off[NV];
 #pragma unroll
-                for (int x = 0; x < NA; ++x)
+                    for (int w = 0; w < ND; ++w)
 #pragma unroll
-                    for (int w = 0; w < ND; ++w) {
-                        double s = 0.0;
+                        for (int x = 0; x < NA; ++x) {
+                            double s = 0.0;
 #pragma unroll
-                        for (int y = 0; y < NB; ++y)
+                            for (int y = 0; y < NB; ++y)
 #pragma unroll
-                            for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dbc[y * NC + z];
-                        s = run_scan(s, lane, run_start);
-                        if (tail && pvalid) atomic_add(VK + (size_t)(d0 + w) * nao + a0 + x, s);
+                                for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dbc[y * NC + z];
+                            val[w * NA + x] = run_scan(s, lane, run_start);
+                            off[w * NA + x] = tail && pvalid ? (d0 + w) * nao + a0 + x : -1;
+                        }
+                    if constexpr (pack_nv(NV) > 0) {
+                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v)
+                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
                     }
+                }
                 // K(b,d): per lane
+                {
+                    constexpr int NV = NB * ND;
+                    double val[NV];
+                    int off[NV];
 #pragma unroll
-                for (int y = 0; y < NB; ++y)
+                    for (int w = 0; w < ND; ++w)
 #pragma unroll
-                    for (int w = 0; w < ND; ++w) {
-                        double s = 0.0;
+                        for (int y = 0; y < NB; ++y) {
+                            double s = 0.0;
 #pragma unroll
-                        for (int x = 0; x < NA; ++x)
+                            for (int x = 0; x < NA; ++x)
 #pragma unroll
-                            for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dac[x * NC + z];
-                        if (active) atomic_add(VK + (size_t)(d0 + w) * nao + b0 + y, s);
+                                for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dac[x * NC + z];
+                            val[w * NB + y] = s;
+                            off[w * NB + y] = active ? (d0 + w) * nao + b0 + y : -1;
+                        }
+                    if constexpr (pack_nv(NV) > 0) {
+                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v)
+                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
                     }
+                }
             }
 #undef G
             // end of a run of equal c: flush K(a,c), K(b,c)
@@ -297,22 +372,47 @@ __global__ void __launch_bounds__(256)
                     ubra[tk.bas_q0 + uperm_q[q + 1]]);
             }
             if (VK && sc_next != sc) {
+                {
+                    constexpr int NV = NA * NC;
+                    double val[NV];
+                    int off[NV];
 #pragma unroll
-                for (int x = 0; x < NA; ++x)
+                    for (int z = 0; z < NC; ++z)
 #pragma unroll
-                    for (int z = 0; z < NC; ++z) {
-                        const double s = run_scan(kac[x * NC + z], lane, run_start);
-                        if (tail && pvalid) atomic_add(VK + (size_t)(c0 + z) * nao + a0 + x, s);
-                        kac[x * NC + z] = 0.0;
+                        for (int x = 0; x < NA; ++x) {
+                            val[z * NA + x] = run_scan(kac[x * NC + z], lane, run_start);
+                            off[z * NA + x] = tail && pvalid ? (c0 + z) * nao + a0 + x : -1;
+                            kac[x * NC + z] = 0.0;
+                        }
+                    if constexpr (pack_nv(NV) > 0) {
+                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v)
+                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
                     }
+                }
+                {
+                    constexpr int NV = NB * NC;
+                    double val[NV];
+                    int off[NV];
 #pragma unroll
-                for (int y = 0; y < NB; ++y)
+                    for (int z = 0; z < NC; ++z)
 #pragma unroll
-                    for (int z = 0; z < NC; ++z) {
-                        if (pvalid && kbc[y * NC + z] != 0.0)
-                            atomic_add(VK + (size_t)(c0 + z) * nao + b0 + y, kbc[y * NC + z]);
-                        kbc[y * NC + z] = 0.0;
+                        for (int y = 0; y < NB; ++y) {
+                            val[z * NB + y] = kbc[y * NC + z];
+                            off[z * NB + y] =
+                                pvalid && kbc[y * NC + z] != 0.0 ? (c0 + z) * nao + b0 + y : -1;
+                            kbc[y * NC + z] = 0.0;
+                        }
+                    if constexpr (pack_nv(NV) > 0) {
+                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v)
+                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
                     }
+                }
             }
             new_c = sc_next != sc;
             sc = sc_next;
