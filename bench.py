@@ -183,10 +183,15 @@ def run_single(args, torch):
         log(f'autotuned {len(table)} (kind, class, precision) tiles: {tuned}')
     ref = None
     if args.precision != 'fp64':
+        t1 = time.perf_counter()
         rj, rk = step('fp64')
         ref = (rj.cpu().numpy(), rk.cpu().numpy())
-    for _ in range(args.warmup):
+        log(f'FP64 reference build {time.perf_counter() - t1:.2f} s')
+    for i in range(args.warmup):
+        t1 = time.perf_counter()
         step()
+        torch.cuda.synchronize()
+        log(f'warm-up build {i}: {time.perf_counter() - t1:.3f} s')
     torch.cuda.synchronize()
     hf.profile_read()
     hf.profile_enable(True)
@@ -197,6 +202,8 @@ def run_single(args, torch):
         vj, vk = step()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
+        if lat[-1] > 2.0:
+            log(f'build {len(lat) - 1}: {lat[-1]:.3f} s')
     total = time.perf_counter() - t_start
     hf.profile_enable(False)
     recs = hf.profile_read()

@@ -201,9 +201,9 @@ __device__ __forceinline__ void digest(const T *__restrict__ gout, const JKArgs 
     const int nao = jk.nao;
     const size_t nao2 = (size_t)nao * nao;
     for (int idm = 0; idm < jk.n_dm; ++idm) {
-        const double *__restrict__ D = jk.dm + idm * nao2;
+        const gptr<double> D = glb(jk.dm + idm * nao2);
         if (jk.vj) {
-            double *V = jk.vj + idm * nao2;
+            gdouble *V = glb_rw(jk.vj + idm * nao2);
 #pragma unroll C::UO
             for (int ia = 0; ia < XN; ++ia)
 #pragma unroll
@@ -232,7 +232,7 @@ __device__ __forceinline__ void digest(const T *__restrict__ gout, const JKArgs 
                 }
         }
         if (jk.vk) {
-            double *V = jk.vk + idm * nao2;
+            gdouble *V = glb_rw(jk.vk + idm * nao2);
             // V[a,c] += g D[b,d]; V[a,d] += g D[b,c]
 #pragma unroll C::UO
             for (int ia = 0; ia < XN; ++ia) {
@@ -311,11 +311,13 @@ __global__ void __launch_bounds__(BQC_THREADS)
     const int bq = ta.swap ? bas_ij : bas_kl;
     const int pp0 = ta.prim_p0 + (bp - ta.bas_p0) * ta.nprim_p;
     const int qq0 = ta.prim_q0 + (bq - ta.bas_q0) * ta.nprim_q;
-    const int sa = pd.bra[bp], sb = pd.ket[bp], sc = pd.bra[bq], sd = pd.ket[bq];
-    const double ra[3] = {pd.bas_x[sa], pd.bas_y[sa], pd.bas_z[sa]};
-    const double rb[3] = {pd.bas_x[sb], pd.bas_y[sb], pd.bas_z[sb]};
-    const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
-    const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
+    const gptr<int> bra = glb(pd.bra), ket = glb(pd.ket), ao = glb(pd.ao_loc);
+    const gptr<double> bx = glb(pd.bas_x), by = glb(pd.bas_y), bz = glb(pd.bas_z);
+    const int sa = bra[bp], sb = ket[bp], sc = bra[bq], sd = ket[bq];
+    const double ra[3] = {bx[sa], by[sa], bz[sa]};
+    const double rb[3] = {bx[sb], by[sb], bz[sb]};
+    const double rc[3] = {bx[sc], by[sc], bz[sc]};
+    const double rd[3] = {bx[sd], by[sd], bz[sd]};
     using C = Cls<LA, LB, LC, LD>;
     if constexpr (C::BIG) {
         // one Cartesian component of shell a at a time keeps the block small
@@ -323,15 +325,13 @@ __global__ void __launch_bounds__(BQC_THREADS)
             T gout[C::NF / C::NFI];
             eri_block<LA, LB, LC, LD, T, 1>(gout, pd, pp0, ta.nprim_p, qq0, ta.nprim_q, norm, ra,
                                             rb, rc, rd, x0);
-            digest<LA, LB, LC, LD, T, 1>(gout, jk, pd.ao_loc[sa], pd.ao_loc[sb], pd.ao_loc[sc],
-                                         pd.ao_loc[sd], x0);
+            digest<LA, LB, LC, LD, T, 1>(gout, jk, ao[sa], ao[sb], ao[sc], ao[sd], x0);
         }
     } else {
         T gout[C::NF];
         eri_block<LA, LB, LC, LD, T>(gout, pd, pp0, ta.nprim_p, qq0, ta.nprim_q, norm, ra, rb, rc,
                                      rd);
-        digest<LA, LB, LC, LD, T>(gout, jk, pd.ao_loc[sa], pd.ao_loc[sb], pd.ao_loc[sc],
-                                  pd.ao_loc[sd]);
+        digest<LA, LB, LC, LD, T>(gout, jk, ao[sa], ao[sb], ao[sc], ao[sd]);
     }
 }
 

@@ -58,19 +58,25 @@ __global__ void __launch_bounds__(64)
     const TaskDesc &tk = tasks[it.x];
     const PairData &pd = tk.pd;
     const JKArgs jk = {tk.vj, tk.vk, tk.dm, tk.nao, tk.n_dm};
+    // everything but the density gathers is the same for all lanes: scalar loads
+    const cptr<int> ubra = cst(pd.bra), uket = cst(pd.ket), uao = cst(pd.ao_loc);
+    const cptr<int> uperm_q = cst(tk.perm_q);
+    const cptr<double> ubx = cst(pd.bas_x), uby = cst(pd.bas_y), ubz = cst(pd.bas_z);
+    const cptr<double> ua12 = cst(pd.a12), ue12 = cst(pd.e12), ux12 = cst(pd.x12),
+                       uy12 = cst(pd.y12), uz12 = cst(pd.z12);
     const int pidx = it.y;
     int s = 0;
     while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
     const int q0 = it.z;
     const int q1 = min(it.w, tk.qbins[tk.nbins - s]);
     if (q1 <= q0) return;
-    const int ppos = tk.perm_p[pidx];
+    const int ppos = cst(tk.perm_p)[pidx];
     const int gp = tk.bas_p0 + ppos;
-    const int sa = pd.bra[gp], sb = pd.ket[gp];
+    const int sa = ubra[gp], sb = uket[gp];
     const int pp0 = tk.prim_p0 + ppos * tk.nprim_p;
-    const int a0 = pd.ao_loc[sa], b0 = pd.ao_loc[sb];
-    const double ra[3] = {pd.bas_x[sa], pd.bas_y[sa], pd.bas_z[sa]};
-    const double rb[3] = {pd.bas_x[sb], pd.bas_y[sb], pd.bas_z[sb]};
+    const int a0 = uao[sa], b0 = uao[sb];
+    const double ra[3] = {ubx[sa], uby[sa], ubz[sa]};
+    const double rb[3] = {ubx[sb], uby[sb], ubz[sb]};
     const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
     // I-table indices of this lane's components
     int gi[W::NG][3];
@@ -83,9 +89,9 @@ __global__ void __launch_bounds__(64)
     const int nao = jk.nao;
     const size_t nao2 = (size_t)nao * nao;
     for (int idm = 0; idm < jk.n_dm; ++idm) {
-        const double *__restrict__ D = jk.dm + idm * nao2;
-        double *VJ = jk.vj ? jk.vj + idm * nao2 : nullptr;
-        double *VK = jk.vk ? jk.vk + idm * nao2 : nullptr;
+        const gptr<double> D = glb(jk.dm + idm * nao2);
+        gdouble *VJ = jk.vj ? glb_rw(jk.vj + idm * nao2) : nullptr;
+        gdouble *VK = jk.vk ? glb_rw(jk.vk + idm * nao2) : nullptr;
         for (int e = lane; e < W::NAB; e += 64)
             sDab[e] = D[(size_t)(a0 + e / NB) * nao + b0 + e % NB];
         double jab[W::EAB], kac[W::EAC], kbc[W::EBC];
@@ -95,12 +101,12 @@ __global__ void __launch_bounds__(64)
         for (int m = 0; m < W::EAC; ++m) kac[m] = 0.0;
 #pragma unroll
         for (int m = 0; m < W::EBC; ++m) kbc[m] = 0.0;
-        int sc = pd.bra[tk.bas_q0 + tk.perm_q[q0]];
+        int sc = ubra[tk.bas_q0 + uperm_q[q0]];
         for (int q = q0; q < q1; ++q) {
-            const int qpos = tk.perm_q[q];
+            const int qpos = uperm_q[q];
             const int gq = tk.bas_q0 + qpos;
-            const int sd = pd.ket[gq];
-            const int c0 = pd.ao_loc[sc], d0 = pd.ao_loc[sd];
+            const int sd = uket[gq];
+            const int c0 = uao[sc], d0 = uao[sd];
             const bool skip = tk.diag && ppos < qpos;
             double norm = TWO_PI_52;
             if (tk.diag && ppos == qpos) norm *= 0.5;
@@ -108,18 +114,18 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int m = 0; m < W::NG; ++m) g[m] = (T)0;
             if (!skip) {
-                const double rc[3] = {pd.bas_x[sc], pd.bas_y[sc], pd.bas_z[sc]};
-                const double rd[3] = {pd.bas_x[sd], pd.bas_y[sd], pd.bas_z[sd]};
+                const double rc[3] = {ubx[sc], uby[sc], ubz[sc]};
+                const double rd[3] = {ubx[sd], uby[sd], ubz[sd]};
                 const double cd[3] = {rc[0] - rd[0], rc[1] - rd[1], rc[2] - rd[2]};
                 const int qq0 = tk.prim_q0 + qpos * tk.nprim_q;
                 for (int ip = 0; ip < tk.nprim_p; ++ip) {
                     const int p = pp0 + ip;
-                    const double aij = pd.a12[p], eij = pd.e12[p];
-                    const double P[3] = {pd.x12[p], pd.y12[p], pd.z12[p]};
+                    const double aij = ua12[p], eij = ue12[p];
+                    const double P[3] = {ux12[p], uy12[p], uz12[p]};
                     for (int iq = 0; iq < tk.nprim_q; ++iq) {
                         const int qq = qq0 + iq;
-                        const double akl = pd.a12[qq], ekl = pd.e12[qq];
-                        const double Q[3] = {pd.x12[qq], pd.y12[qq], pd.z12[qq]};
+                        const double akl = ua12[qq], ekl = ue12[qq];
+                        const double Q[3] = {ux12[qq], uy12[qq], uz12[qq]};
                         const double pq[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
                         const double aijkl = aij + akl, a1 = aij * akl, a0q = a1 / aijkl;
                         const double x = a0q * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]);
@@ -252,7 +258,7 @@ __global__ void __launch_bounds__(64)
 #undef SG
             __syncthreads();
             // end of a run of equal c: flush K(a,c), K(b,c)
-            const int sc_next = q + 1 < q1 ? pd.bra[tk.bas_q0 + tk.perm_q[q + 1]] : -1;
+            const int sc_next = q + 1 < q1 ? ubra[tk.bas_q0 + uperm_q[q + 1]] : -1;
             if (VK && sc_next != sc) {
 #pragma unroll
                 for (int m = 0; m < W::EAC; ++m) {
