@@ -35,6 +35,8 @@ _SIGNATURES = {
     'BQCgraph_capture': (c_void_p, [c_int] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p,
                                     c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
                                     c_double, c_void_p]),
+    'GINTfill_int2e': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_int, c_int, c_int, c_void_p]),
     'BQCgraph_launch': (c_int, [c_void_p, c_void_p]),
     'BQCgraph_destroy': (None, [c_void_p]),
     'BQCset_tile': (c_int, [c_int, c_int, c_int, c_int]),

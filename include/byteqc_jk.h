@@ -93,6 +93,19 @@ int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, in
 
 /* ---- entry points of this library only ---------------------------------- */
 
+/* Four-index ERIs of the screened quartets of pair types cp_ij_id (bra)
+ * and cp_kl_id (ket), added to eri (device) at
+ *   (i - ao_offsets[0]) strides[0] + (j - ao_offsets[1]) strides[1]
+ *   + (k - ao_offsets[2]) strides[2] + (l - ao_offsets[3]) strides[3]
+ * for the Cartesian AOs i, j (pair bra, ket) and k, l of every quartet; kl
+ * bin b meets the ij pairs of bins [0, nbins - b) (host bin offsets as for
+ * GINTbuild_jk).  Same name, arguments and return codes (0 ok, 1 launch
+ * error / unsupported class, 2 nroots >= 12) as the reference
+ * (cuobc/lib/gint/nr_fill_ao_ints.cu:493-550). */
+int GINTfill_int2e(BasisProdCache *bpcache, double *eri, int nao, const int *strides,
+                   const int *ao_offsets, const int *bins_locs_ij, const int *bins_locs_kl,
+                   int nbins, int cp_ij_id, int cp_kl_id, hipStream_t stream);
+
 /* Frees a clone made by GINTinit_basis_prod_gpu (the reference leaks it). */
 void BQCdel_basis_prod_clone(BasisProdCache **pbp);
 
