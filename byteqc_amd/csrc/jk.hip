@@ -342,7 +342,8 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, int wave, const int *bra_
         k32 = c.swap ? g_rows[1][c.pkl][c.pij] : g_rows[1][c.pij][c.pkl];
     }
     if (!k64 || !k32 || !bra_order) return -1;
-    if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32)) return 1;
+    if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32))
+        return wave ? 1 : -1;  // a row kernel that spills too much gives way to the wave kernel
     BasisProdCache *bp = c.bp;
     const int nbins = c.nbins;
     const int *pbins = c.swap ? c.bins_kl : c.bins_ij;

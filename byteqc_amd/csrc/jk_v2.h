@@ -437,10 +437,13 @@ static void reg_row(RowKernelTable &tab) {
     tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, float>;
 }
 
-// Classes digested with the wavefront-stationary kernel (register budget:
-// the integral block and the lane accumulators must stay in VGPRs).
+// Classes digested with the wavefront-stationary kernel.  Up to 108
+// Cartesian components the block and the lane accumulators fit the 512
+// registers of one wave per SIMD (with small spills), and this kernel beats
+// the wavefront-per-quartet one by ~3x (measured on (dp|ds)); at 162 the
+// spills make it slower than that kernel.
 template <int LA, int LB, int LC, int LD>
 constexpr bool use_rows() {
-    return Cls<LA, LB, LC, LD>::NF <= 81;
+    return Cls<LA, LB, LC, LD>::NF <= 108;
 }
 
