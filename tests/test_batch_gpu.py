@@ -89,6 +89,7 @@ def test_batch_mixed_precision_energy():
         eb = 0.5 * numpy.sum(d * (b[0].cpu().numpy() - 0.5 * b[1].cpu().numpy()))
         assert abs(ea - eb) < 1e-8
         assert _maxdiff(a[0].cpu(), b[0].cpu()) < 1e-8
+        assert _maxdiff(a[1].cpu(), b[1].cpu()) < 1e-8
 
 
 def test_autotuned_tiles_keep_results():

@@ -163,3 +163,22 @@ def test_rhf_known_answer_and_mixed_schedule():
     emx, mf2 = rhf.scf_energy(mol, conv_tol=1e-10, precision='mixed_then_fp64')
     print(f'RHF mixed->fp64 {emx:.10f} builds {mf2.jk_builds}', flush=True)
     assert mf2.converged and abs(emx - e64) < 1e-8
+
+
+def test_fp32_everywhere_cc_pvtz_stays_at_fp32_accuracy():
+    """All quartets in FP32 (cut 1e30), every class to (ff|ff): J and K stay
+    within FP32 rounding of the FP64 result (catches a broken FP32 kernel
+    that the 1e-7 split would hide behind small integrals)."""
+    _torch_gpu()
+    from byteqc_amd.gto import M
+    from byteqc_amd.scf import hf
+    mol = M(atom='O 0 0 0.58589194; H 0 0.75668992 0; H 0 -0.75668992 0', basis='cc-pvtz')
+    rng = numpy.random.default_rng(5)
+    a = rng.standard_normal((mol.nao, mol.nao))
+    dm = a + a.T
+    opt = hf._VHFOpt(mol).build()
+    j0, k0 = hf.get_jk(mol, dm, vhfopt=opt)
+    j1, k1 = hf.get_jk(mol, dm, vhfopt=opt, precision='mixed', fp32_cut=1e30)
+    dj, dk = _report('1 H2O cc-pVTZ all FP32', j1, k1, j0, k0)
+    scale = max(abs(j0).max(), abs(k0).max())
+    assert dj < 1e-5 * scale and dk < 1e-5 * scale
