@@ -353,9 +353,11 @@ def run_serve(args, torch):
     # warm-up request (kernel loading), not counted
     srv.submit(water_mole(3, box_for_density(3), 1, 'def2-svp'))
     srv.drain()
+    srv.close()
     srv = Server(log=log)
     reqs, wall = run_load(srv, arr, lambda n, sd: water_mole(n, box_for_density(n), sd,
                                                             'def2-svp'), log=log)
+    srv.close()
     lat = [r.latency for r in reqs]
     ok = [r for r in reqs if r.result and r.result.get('converged')]
     st = srv.stats
@@ -379,7 +381,9 @@ def run_serve(args, torch):
                     'mean_batch': float(numpy.mean(st['batch_sizes'])),
                     'max_active': st['max_active'],
                     'max_hbm_in_use_gib': st['max_in_use'] / 2**30,
-                    'hbm_budget_gib': srv.hbm_budget / 2**30},
+                    'hbm_budget_gib': srv.hbm_budget / 2**30,
+                    'mean_scf_cycles': float(numpy.mean([r.jk_builds for r in reqs])),
+                    'loop_seconds': {k: round(v, 3) for k, v in st['seconds'].items()}},
         'roofline': None,
     }
 

@@ -9,6 +9,8 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "jk_wave.h"
@@ -18,7 +20,6 @@ static int pair_id(int a, int b) { return a * (a + 1) / 2 + b; }
 static JKKernelTable g_kernels;
 static RowKernelTable g_rows;
 static WaveKernelTable g_waves;
-static bool g_registered = false;
 
 #define BQC_DECL_BRA(LA, LB) \
     void bqc_register_bra_##LA##LB(JKKernelTable &t1, RowKernelTable &t2, WaveKernelTable &t3);
@@ -33,8 +34,7 @@ BQC_DECL_BRA(3, 1)
 BQC_DECL_BRA(3, 2)
 BQC_DECL_BRA(3, 3)
 
-static void register_kernels() {
-    if (g_registered) return;
+static void register_kernels_once() {
     bqc_register_bra_00(g_kernels, g_rows, g_waves);
     bqc_register_bra_10(g_kernels, g_rows, g_waves);
     bqc_register_bra_11(g_kernels, g_rows, g_waves);
@@ -45,10 +45,13 @@ static void register_kernels() {
     bqc_register_bra_31(g_kernels, g_rows, g_waves);
     bqc_register_bra_32(g_kernels, g_rows, g_waves);
     bqc_register_bra_33(g_kernels, g_rows, g_waves);
-    g_registered = true;
 }
+static std::once_flag g_register_once;
+static void register_kernels() { std::call_once(g_register_once, register_kernels_once); }
 
-static double g_last_flops[2] = {0, 0};
+
+// FLOPs of the last build made by the calling thread.
+static thread_local double g_last_flops[2] = {0, 0};
 
 // Optional per-launch timing: HIP events around every kernel of a build,
 // on the build's stream.  Records (class code, algorithmic FLOPs, events).
@@ -60,6 +63,7 @@ struct LaunchRec {
     hipEvent_t e0, e1;
 };
 static bool g_prof = false;
+static std::mutex g_prof_mu;  // guards g_recs and g_evpool
 static std::vector<LaunchRec> g_recs;
 static std::vector<hipEvent_t> g_evpool;
 
@@ -155,9 +159,18 @@ static void record_launch(LaunchRec &rec, int cls, int kind, double flops, doubl
     rec.kind = kind;
     rec.flops = flops;
     rec.nquart = nq;
-    rec.e0 = prof_event();
-    rec.e1 = prof_event();
+    {
+        std::lock_guard<std::mutex> lock(g_prof_mu);
+        rec.e0 = prof_event();
+        rec.e1 = prof_event();
+    }
     hipEventRecord(rec.e0, stream);
+}
+
+static void keep_launch(const LaunchRec &rec, hipStream_t stream) {
+    hipEventRecord(rec.e1, stream);
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    g_recs.push_back(rec);
 }
 
 static int eval_cls(const TaskCtx &c) {
@@ -168,11 +181,25 @@ static int eval_cls(const TaskCtx &c) {
 // Per-lane scratch (private segment) limit for a launch; the runtime aborts
 // the process when it cannot back a kernel's scratch, so oversized kernels
 // are refused here instead.
+// A kernel whose static LDS exceeds the device's per-workgroup limit is
+// refused the same way.  Answers are cached per kernel.
 #define BQC_MAX_SCRATCH_BYTES 32768
 static bool scratch_ok(const void *kern) {
+    static std::mutex mu;
+    static std::map<const void *, bool> known;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = known.find(kern);
+    if (it != known.end()) return it->second;
     hipFuncAttributes attr;
-    if (hipFuncGetAttributes(&attr, kern) != hipSuccess) return false;
-    return attr.localSizeBytes <= BQC_MAX_SCRATCH_BYTES;
+    bool ok = hipFuncGetAttributes(&attr, kern) == hipSuccess &&
+              attr.localSizeBytes <= BQC_MAX_SCRATCH_BYTES;
+    int dev = 0, lds_max = 0;
+    if (ok && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+            hipSuccess)
+        ok = attr.sharedSizeBytes <= (size_t)lds_max;
+    known[kern] = ok;
+    return ok;
 }
 
 // Generic kernels (one lane per quartet): rectangle kl-bin b x ij in
@@ -240,10 +267,7 @@ static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &p
         LaunchRec rec;
         if (g_prof) record_launch(rec, eval_cls(c), 2 * prec, lflops, nq[prec], stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, t);
-        if (g_prof) {
-            hipEventRecord(rec.e1, stream);
-            g_recs.push_back(rec);
-        }
+        if (g_prof) keep_launch(rec, stream);
         hipError_t err = hipGetLastError();
         if (err != hipSuccess) {
             fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", c.li, c.lj, c.lk, c.ll,
@@ -329,21 +353,28 @@ static void side_order(const int *bins, int nbins, const int *order, int npairs_
     }
 }
 
-// Adds one task to the plan with row (wave == 0) or wave (wave == 1)
-// kernels.  Returns -1 if the class has no such kernel.
-static int plan_task(BuildPlan &bp_, const TaskCtx &c, int wave, const int *bra_order,
-                     const JKArgs &jk, const PairData &pd) {
-    RowKernel k64, k32;
-    if (wave) {
-        k64 = c.swap ? g_waves[0][c.pkl][c.pij] : g_waves[0][c.pij][c.pkl];
-        k32 = c.swap ? g_waves[1][c.pkl][c.pij] : g_waves[1][c.pij][c.pkl];
-    } else {
-        k64 = c.swap ? g_rows[0][c.pkl][c.pij] : g_rows[0][c.pij][c.pkl];
-        k32 = c.swap ? g_rows[1][c.pkl][c.pij] : g_rows[1][c.pij][c.pkl];
+// Adds one task to the plan.  Each precision takes the row kernel of the
+// class when there is one within the scratch and LDS limits, else the wave
+// kernel.  Returns -1 if a precision has neither (the generic kernels take
+// the task).
+static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, const JKArgs &jk,
+                     const PairData &pd) {
+    if (!bra_order) return -1;
+    RowKernel k[2];
+    int wv[2];
+    for (int prec = 0; prec < 2; ++prec) {
+        RowKernel r = c.swap ? g_rows[prec][c.pkl][c.pij] : g_rows[prec][c.pij][c.pkl];
+        RowKernel w = c.swap ? g_waves[prec][c.pkl][c.pij] : g_waves[prec][c.pij][c.pkl];
+        if (r && scratch_ok((const void *)r)) {
+            k[prec] = r;
+            wv[prec] = 0;
+        } else if (w && scratch_ok((const void *)w)) {
+            k[prec] = w;
+            wv[prec] = 1;
+        } else {
+            return -1;
+        }
     }
-    if (!k64 || !k32 || !bra_order) return -1;
-    if (!scratch_ok((const void *)k64) || !scratch_ok((const void *)k32))
-        return wave ? 1 : -1;  // a row kernel that spills too much gives way to the wave kernel
     BasisProdCache *bp = c.bp;
     const int nbins = c.nbins;
     const int *pbins = c.swap ? c.bins_kl : c.bins_ij;
@@ -400,18 +431,17 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, int wave, const int *bra_
     bp_.perm_p_off.push_back(off_p);
     bp_.perm_q_off.push_back(off_q);
     const int cls = eval_cls(c);
-    Group *g[2] = {&bp_.group(k64, wave, cls, 0), nullptr};
-    g[1] = &bp_.group(k32, wave, cls, 1);
-    g[0] = &bp_.group(k64, wave, cls, 0);  // re-fetch: group() may reallocate
+    Group *g[2] = {&bp_.group(k[0], wv[0], cls, 0), nullptr};
+    g[1] = &bp_.group(k[1], wv[1], cls, 1);
+    g[0] = &bp_.group(k[0], wv[0], cls, 0);  // re-fetch: group() may reallocate
     for (int prec = 0; prec < 2; ++prec) {
         g[prec]->flops += nquart[prec] * c.qflops;
         g[prec]->nquart += nquart[prec];
     }
-    const int ck[2] = {tile_for(wave ? 4 : 1, cls, 0), tile_for(wave ? 4 : 1, cls, 1)};
-    const int step = wave ? 1 : 64;
-    for (int p = 0; p < np; p += step) {
+    // Q end of P position p's bin and the first Q position computed in FP32
+    auto q_limits = [&](int p, int &qmax, int &qsplit) {
         int s = (int)(std::upper_bound(pbins + 1, pbins + nbins + 1, p) - (pbins + 1));
-        int qmax = qbins[nbins - s];
+        qmax = qbins[nbins - s];
         double ubp = bin_ub(c, cp_p, pbins, s);
         int split = nbins - s;
         for (int bq = 0; bq < nbins - s; ++bq)
@@ -419,13 +449,28 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, int wave, const int *bra_
                 split = bq;
                 break;
             }
-        int qsplit = qbins[split];
-        int lim[2][2] = {{0, qsplit < qmax ? qsplit : qmax}, {qsplit, qmax}};
-        const int pitem = wave ? p : p / 64;
-        for (int prec = 0; prec < 2; ++prec) {
-            std::vector<int> &it = g[prec]->items;
-            for (int q0 = lim[prec][0]; q0 < lim[prec][1]; q0 += ck[prec]) {
-                int q1 = q0 + ck[prec] < lim[prec][1] ? q0 + ck[prec] : lim[prec][1];
+        qsplit = qbins[split];
+    };
+    // When one precision runs rows of 64 P pairs, the split of a row's first
+    // pair holds for every pair of the row in both precisions, so the two
+    // precisions cover each pair's Q range exactly once.
+    const bool by_row = !wv[0] || !wv[1];
+    for (int prec = 0; prec < 2; ++prec) {
+        const int ck = tile_for(wv[prec] ? 4 : 1, cls, prec);
+        const int step = wv[prec] ? 1 : 64;
+        std::vector<int> &it = g[prec]->items;
+        for (int p = 0; p < np; p += step) {
+            int qmax, qsplit;
+            q_limits(p, qmax, qsplit);
+            if (by_row && (p & 63)) {
+                int rmax;
+                q_limits(p & ~63, rmax, qsplit);
+            }
+            const int lo = prec ? qsplit : 0;
+            const int hi = prec ? qmax : (qsplit < qmax ? qsplit : qmax);
+            const int pitem = wv[prec] ? p : p / 64;
+            for (int q0 = lo; q0 < hi; q0 += ck) {
+                int q1 = q0 + ck < hi ? q0 + ck : hi;
                 it.push_back(ti);
                 it.push_back(pitem);
                 it.push_back(q0);
@@ -483,10 +528,7 @@ static int launch_groups(const BuildPlan &p, const Staged &st, char *dev, hipStr
         LaunchRec rec;
         if (g_prof) record_launch(rec, g.cls, (g.wave ? 4 : 1) + 2 * g.prec, g.flops, g.nquart, stream);
         hipLaunchKernelGGL(g.kern, dim3(nblk), dim3(nthr), 0, stream, tasks, items, nitems);
-        if (g_prof) {
-            hipEventRecord(rec.e1, stream);
-            g_recs.push_back(rec);
-        }
+        if (g_prof) keep_launch(rec, stream);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             fprintf(stderr, "bqcjk: kernel launch failed: %s\n", hipGetErrorString(e));
@@ -496,42 +538,65 @@ static int launch_groups(const BuildPlan &p, const Staged &st, char *dev, hipStr
     return 0;
 }
 
-// Shared device image for direct (non-graph) builds.
-static char *g_dws = nullptr;
-static size_t g_dws_cap = 0;
-static char *g_hstage = nullptr;
-static size_t g_hstage_cap = 0;
-static hipEvent_t g_stage_evt = nullptr;
-static bool g_stage_pending = false;
+// Device image of the work list of direct (non-graph) builds, one per
+// stream.  Builds on one stream run in order, so a build may refill the
+// image once its copy is queued behind the previous build's kernels; builds
+// on different streams have separate images and may overlap on the device.
+// One host thread drives a given stream at a time.
+struct StreamStage {
+    char *dws = nullptr;
+    size_t dws_cap = 0;
+    char *hstage = nullptr;
+    size_t hstage_cap = 0;
+    hipEvent_t evt = nullptr;
+    bool pending = false;
+};
+static std::mutex g_stage_mu;
+static std::map<hipStream_t, StreamStage> g_stages;
+
+static StreamStage &stage_for(hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    return g_stages[stream];
+}
 
 static int stage_and_launch(BuildPlan &p, hipStream_t stream) {
     if (p.groups.empty()) return 0;
     Staged st = stage_layout(p);
-    if (g_stage_pending) {
-        hipEventSynchronize(g_stage_evt);
-        g_stage_pending = false;
+    StreamStage &ss = stage_for(stream);
+    if (ss.pending) {
+        // the previous copy from the pinned buffer must be done before refilling it
+        hipEventSynchronize(ss.evt);
+        ss.pending = false;
     }
-    if (st.bytes > g_hstage_cap) {
-        if (g_hstage) hipHostFree(g_hstage);
+    if (st.bytes > ss.hstage_cap) {
+        if (ss.hstage) hipHostFree(ss.hstage);
         size_t cap = st.bytes + st.bytes / 2 + 4096;
-        if (hipHostMalloc((void **)&g_hstage, cap, hipHostMallocDefault) != hipSuccess) return 1;
-        g_hstage_cap = cap;
+        if (hipHostMalloc((void **)&ss.hstage, cap, hipHostMallocDefault) != hipSuccess) {
+            ss.hstage = nullptr;
+            ss.hstage_cap = 0;
+            return 1;
+        }
+        ss.hstage_cap = cap;
     }
-    if (st.bytes > g_dws_cap) {
-        // kernels queued earlier may still read the old image
+    if (st.bytes > ss.dws_cap) {
+        // kernels queued earlier on this stream may still read the old image
         hipStreamSynchronize(stream);
-        if (g_dws) hipFree(g_dws);
+        if (ss.dws) hipFree(ss.dws);
         size_t cap = st.bytes + st.bytes / 2 + 4096;
-        if (hipMalloc((void **)&g_dws, cap) != hipSuccess) return 1;
-        g_dws_cap = cap;
+        if (hipMalloc((void **)&ss.dws, cap) != hipSuccess) {
+            ss.dws = nullptr;
+            ss.dws_cap = 0;
+            return 1;
+        }
+        ss.dws_cap = cap;
     }
-    if (!g_stage_evt) hipEventCreateWithFlags(&g_stage_evt, hipEventDisableTiming);
-    stage_fill(p, st, g_dws, g_hstage);
-    if (hipMemcpyAsync(g_dws, g_hstage, st.bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
+    if (!ss.evt) hipEventCreateWithFlags(&ss.evt, hipEventDisableTiming);
+    stage_fill(p, st, ss.dws, ss.hstage);
+    if (hipMemcpyAsync(ss.dws, ss.hstage, st.bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
         return 1;
-    hipEventRecord(g_stage_evt, stream);
-    g_stage_pending = true;
-    return launch_groups(p, st, g_dws, stream);
+    hipEventRecord(ss.evt, stream);
+    ss.pending = true;
+    return launch_groups(p, st, ss.dws, stream);
 }
 
 struct MolArgs {
@@ -567,8 +632,7 @@ static int plan_molecules(BuildPlan &plan, const MolArgs *m, int nmol, int nbins
             int err = task_init(c, a.bp, bi, bk, nbins, a.task_cp[2 * t], a.task_cp[2 * t + 1],
                                 a.log_q, a.task_dmax ? a.task_dmax[t] : 0, fp32_cut, precision);
             if (err) return err;
-            err = plan_task(plan, c, 0, a.bra_order, jk, pd);
-            if (err < 0) err = plan_task(plan, c, 1, a.bra_order, jk, pd);
+            err = plan_task(plan, c, a.bra_order, jk, pd);
             if (err < 0 && deferred) {
                 deferred->push_back(Deferred{c, jk, pd});
                 err = 0;
@@ -741,11 +805,15 @@ void BQCprofile_enable(int on) {
     g_prof = on != 0;
 }
 
-int BQCprofile_count(void) { return (int)g_recs.size(); }
+int BQCprofile_count(void) {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    return (int)g_recs.size();
+}
 
 // Drains the launch records: cls, kind, flops, quartets and duration (ms)
 // per launch; waits for the recorded events.  Returns the number written.
 int BQCprofile_read(int *cls, int *kind, double *flops, double *nquart, double *ms, int cap) {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
     int n = 0;
     for (LaunchRec &r : g_recs) {
         if (n < cap) {

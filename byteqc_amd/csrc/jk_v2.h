@@ -431,19 +431,23 @@ __global__ void __launch_bounds__(256)
 typedef void (*RowKernel)(const TaskDesc *, const int4 *, int);
 typedef RowKernel RowKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
 
-template <int LA, int LB, int LC, int LD>
-static void reg_row(RowKernelTable &tab) {
-    tab[0][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, double>;
-    tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, float>;
-}
-
-// Classes digested with the wavefront-stationary kernel.  Up to 108
-// Cartesian components the block and the lane accumulators fit the 512
-// registers of one wave per SIMD (with small spills), and this kernel beats
-// the wavefront-per-quartet one by ~3x (measured on (dp|ds)); at 162 the
-// spills make it slower than that kernel.
-template <int LA, int LB, int LC, int LD>
+// Classes digested with the wavefront-stationary kernel, per precision.
+// Up to 108 Cartesian components the FP64 block and the lane accumulators
+// fit the 512 registers of one wave per SIMD (with small spills), and this
+// kernel beats the wavefront-per-quartet one by ~3x (measured on (dp|ds));
+// at 162 the spills make it slower than that kernel.  The FP32 instance has
+// the same limit: at 162-216 components (2-3 KB of scratch per lane) it ran
+// at 1.2-1.4 TFLOP/s against 2.4-3.5 for the wavefront-per-quartet kernel.
+template <int LA, int LB, int LC, int LD, typename T>
 constexpr bool use_rows() {
     return Cls<LA, LB, LC, LD>::NF <= 108;
+}
+
+template <int LA, int LB, int LC, int LD>
+static void reg_row(RowKernelTable &tab) {
+    if constexpr (use_rows<LA, LB, LC, LD, double>())
+        tab[0][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, double>;
+    if constexpr (use_rows<LA, LB, LC, LD, float>())
+        tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, float>;
 }
 

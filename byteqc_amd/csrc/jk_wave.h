@@ -292,11 +292,13 @@ __global__ void __launch_bounds__(64)
 typedef RowKernel WaveKernel;
 typedef WaveKernel WaveKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
 
-// Classes digested with the wavefront-per-quartet kernel (LDS budget).
+// Classes digested with the wavefront-per-quartet kernel (LDS budget: up to
+// (fd|fd) and (ff|fd), whose ~80 KB leave room for two workgroups per CU;
+// the planner refuses a kernel above the device's per-workgroup LDS).
 template <int LA, int LB, int LC, int LD>
 constexpr bool use_wave() {
     using C = Cls<LA, LB, LC, LD>;
-    return C::NF > 81 && (3 * C::NR * C::GSZ + C::NF) * 8 <= 40000;
+    return C::NF > 81 && (3 * C::NR * C::GSZ + C::NF) * 8 <= 80000;
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -310,7 +312,7 @@ static void reg_wave(WaveKernelTable &tab) {
 template <int LA, int LB, int LC, int LD>
 static void reg_class(JKKernelTable &t1, RowKernelTable &t2, WaveKernelTable &t3) {
     reg_kernel<LA, LB, LC, LD>(t1);
-    if constexpr (use_rows<LA, LB, LC, LD>()) reg_row<LA, LB, LC, LD>(t2);
+    reg_row<LA, LB, LC, LD>(t2);
     reg_wave<LA, LB, LC, LD>(t3);
 }
 

@@ -47,10 +47,100 @@ def int1e(opt, stream=None):
     return tuple(c.T @ m @ c for m in mats)
 
 
+def _frac_occ(e, nelec, degen=1e-6):
+    """Aufbau occupations (spin-summed, at most 2 per orbital) for orbital
+    energies e; a degenerate set at the Fermi level shares its electrons
+    equally, which keeps an atom's density spherical."""
+    order = numpy.argsort(e)
+    occ = numpy.zeros(len(e))
+    left, i = float(nelec), 0
+    while left > 1e-12 and i < len(e):
+        j = i + 1
+        while j < len(e) and e[order[j]] - e[order[i]] < degen:
+            j += 1
+        put = min(2.0 * (j - i), left)
+        occ[order[i:j]] = put / (j - i)
+        left -= put
+        i = j
+    return occ
+
+
+_ATOM_DM = {}
+
+
+def atomic_density(symbol, basis, max_cycle=60, conv_tol=1e-9):
+    """Spin-restricted, spherically averaged density of a neutral atom in
+    the given basis (spherical contracted AOs, host array): fractional-
+    occupation Hartree-Fock with DIIS, J/K from the GPU engine.  Cached per
+    (element, basis)."""
+    from byteqc_amd.gto import M
+    key = (symbol, basis if isinstance(basis, str) else repr(basis))
+    if key in _ATOM_DM:
+        return _ATOM_DM[key]
+    atom = M(atom=f'{symbol} 0 0 0', basis=basis)
+    opt = hf._VHFOpt(atom).build()
+    s, t, v = (m.cpu().numpy() for m in int1e(opt))
+    h = t + v
+    e, u = numpy.linalg.eigh(s)
+    x = u[:, e > 1e-9] / numpy.sqrt(e[e > 1e-9])
+    nelec = atom.nelectron
+
+    def density(f):
+        eps, c = numpy.linalg.eigh(x.T @ f @ x)
+        c = x @ c
+        return (c * _frac_occ(eps, nelec)) @ c.T
+
+    dm = density(h)
+    fs, errs, e_prev = [], [], None
+    for _ in range(max_cycle):
+        j, k = hf.get_jk(atom, dm, vhfopt=opt)
+        f = h + j - 0.5 * k
+        e_tot = 0.5 * float(numpy.sum(dm * (h + f)))
+        sdf = s @ dm @ f
+        err = x.T @ (sdf.T - sdf) @ x
+        if e_prev is not None and abs(e_tot - e_prev) < conv_tol and abs(err).max() < 1e-5:
+            break
+        e_prev = e_tot
+        fs.append(f)
+        errs.append(err.ravel())
+        fs, errs = fs[-8:], errs[-8:]
+        n = len(fs)
+        if n > 1:
+            b = -numpy.ones((n + 1, n + 1))
+            b[n, n] = 0
+            b[:n, :n] = numpy.array(errs) @ numpy.array(errs).T
+            rhs = numpy.zeros(n + 1)
+            rhs[n] = -1
+            coef = numpy.linalg.lstsq(b, rhs, rcond=None)[0][:n]
+            f = numpy.einsum('i,ijk->jk', coef, numpy.array(fs))
+        dm = density(f)
+    opt.clear()
+    _ATOM_DM[key] = dm
+    return dm
+
+
+def sad_density(mol):
+    """Superposition of atomic densities: block-diagonal over the atoms'
+    AO ranges (host array)."""
+    nao = mol.nao_nr()
+    dm = numpy.zeros((nao, nao))
+    for ia, (_, _, a0, a1) in enumerate(mol.aoslice_by_atom()):
+        sym = mol.atom_symbol(ia)
+        basis = mol.basis[sym] if isinstance(mol.basis, dict) else mol.basis
+        dm[a0:a1, a0:a1] = atomic_density(sym, basis)
+    return dm
+
+
 class RHF:
+    """init_guess: 'atom' (superposition of spherically averaged atomic
+    densities, as PySCF's init_guess='atom') or 'hcore' (core Hamiltonian)."""
+
     def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=80, diis_space=8,
                  precision='fp64', switch_de=1e-5, fp32_cut=1e-7, lindep=1e-9, vhfopt=None,
-                 stream=None):
+                 stream=None, init_guess='atom'):
+        if init_guess not in ('atom', 'hcore'):
+            raise ValueError(f'unknown initial guess {init_guess!r}')
+        self.init_guess = init_guess
         self.mol = mol
         self.conv_tol = conv_tol
         self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else conv_tol ** 0.5
@@ -86,7 +176,10 @@ class RHF:
         self.nocc = self.mol.nelectron // 2
         self._f_hist, self._e_hist = [], []
         self.mo_coeff = self._diag(self.H)
-        self.dm = self._make_dm(self.mo_coeff)
+        if self.init_guess == 'atom':
+            self.dm = torch.as_tensor(sad_density(self.mol), device=s.device)
+        else:
+            self.dm = self._make_dm(self.mo_coeff)
         self.e_prev = None
         self.build_time = time.perf_counter() - t0
         return self

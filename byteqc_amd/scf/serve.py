@@ -5,17 +5,24 @@ and each ``get_jk`` call launches per task from Python
 (cuobc/scf/hf.py:53-227, 253-274).  Here many requests share the GPU:
 
 * A request is either one J/K build for a given density ("jk") or a full
-  RHF run to convergence ("scf": core-Hamiltonian guess, DIIS, mixed-
+  RHF run to convergence ("scf": atomic-density guess, DIIS, mixed-
   precision J/K until |dE| < switch_de, then FP64).
 * Admission by HBM footprint: a request leaves the FIFO queue only when its
   estimated device footprint (plan, pair data, SCF matrices, DIIS history,
   J/K buffers) fits in the budget beside the requests in flight; the oldest
   request goes first and is never overtaken, so large molecules do not starve.
-* Continuous batching: every iteration of the loop builds J/K for *all*
+* Initial guess: superposition of atomic densities (cached per element and
+  basis), so an SCF request needs few iterations.
+* Lanes by size: a request goes to a lane by its number of AOs; each lane
+  has its own HIP stream and host thread.  A lane builds J/K for *all* its
   requests in flight in one batched launch set (``BQCbuild_jk_batch``: one
   launch per integral class and kernel kind covering every molecule), then
-  advances each SCF by one step.  Requests that arrive while others run join
-  at the next iteration; finished ones leave and free their memory at once.
+  advances each SCF by one step, and repeats.  The lanes run concurrently on
+  the device and on the host, so small molecules iterate many times while
+  one build of large molecules runs and their latency does not follow the
+  largest request in flight.
+* Continuous batching: requests that arrive while others run join their
+  lane at its next build; finished ones leave and free their memory at once.
 * Per-request state (plan, pair lists, pair data, S, H, X, D, DIIS history)
   stays on the device between iterations.
 
@@ -25,8 +32,10 @@ its result do not depend on which other requests are in flight.
 """
 from __future__ import annotations
 
+import bisect
 import collections
 import itertools
+import threading
 import time
 
 import numpy
@@ -91,15 +100,35 @@ def estimate_footprint(mol, diis_space=8):
     return int(mats + pairs)
 
 
+class _Lane:
+    """Requests of one size range, their stream and their host thread."""
+
+    def __init__(self):
+        self.stream = None
+        self.reqs = []
+        self.thread = None
+
+
 class Server:
-    """Continuous-batching request loop on one GPU."""
+    """Continuous-batching request loop on one GPU.
+
+    lanes: AO counts separating the size lanes (ascending); a request with
+    nao AOs joins lane bisect_right(lanes, nao).  Each lane runs its builds
+    and SCF steps in a host thread of its own on its own stream, so the host
+    work of one lane (planning, diagonalisation) overlaps the builds of the
+    others; ``step()`` admits requests from the calling thread and returns
+    the requests that finished since the previous call."""
 
     def __init__(self, hbm_budget=None, max_active=64, precision='mixed_then_fp64',
-                 conv_tol=1e-8, switch_de=1e-5, max_cycle=80, fp32_cut=1e-7, log=None):
+                 conv_tol=1e-8, switch_de=1e-5, max_cycle=80, fp32_cut=1e-7, log=None,
+                 lanes=(800, 1600)):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * free)
         self.hbm_budget = int(hbm_budget)
+        self.lane_bounds = tuple(sorted(lanes))
+        self.lanes = [_Lane() for _ in range(len(self.lane_bounds) + 1)]
+        self._setup_stream = None
         self.max_active = max_active
         self.scf_defaults = dict(precision=precision, conv_tol=conv_tol, switch_de=switch_de,
                                  max_cycle=max_cycle, fp32_cut=fp32_cut)
@@ -110,8 +139,17 @@ class Server:
         self.in_use = 0
         self._ids = itertools.count()
         self.log = log
+        self._cv = threading.Condition()
+        self._done = []        # finished since the last step()
+        self._error = None     # first exception raised in a lane thread
+        self._closed = False
+        # seconds spent admitting (plans, one-electron integrals, initial
+        # guess), planning and launching batched J/K builds, waiting for them
+        # on the device and in the SCF steps; lane times add up over threads
         self.stats = {'iterations': 0, 'batched_builds': 0, 'molecule_builds': 0,
-                      'max_active': 0, 'max_in_use': 0, 'batch_sizes': []}
+                      'max_active': 0, 'max_in_use': 0, 'batch_sizes': [],
+                      'seconds': {'admit': 0.0, 'launch': 0.0, 'wait': 0.0, 'scf': 0.0},
+                      'lane_builds': [0] * len(self.lanes)}
 
     # -- requests ----------------------------------------------------------------
     def submit(self, mol, kind='scf', dm=None, arrival=None, **scf_kwargs):
@@ -121,33 +159,57 @@ class Server:
             r.error = (f'request needs ~{r.footprint / 2**30:.2f} GiB of HBM, above the '
                        f'budget of {self.hbm_budget / 2**30:.2f} GiB')
             r.finished = time.perf_counter()
-            self.completed.append(r)
+            with self._cv:
+                self.completed.append(r)
             return r
         self.queue.append(r)
         return r
 
     def _admit(self):
-        while self.queue and len(self.active) < self.max_active:
-            r = self.queue[0]
-            if self.active and self.in_use + r.footprint > self.hbm_budget:
-                break
+        if self._closed:
+            raise RuntimeError('server is closed')
+        while self.queue:
+            with self._cv:
+                if len(self.active) >= self.max_active:
+                    break
+                r = self.queue[0]
+                if self.active and self.in_use + r.footprint > self.hbm_budget:
+                    break
             self.queue.popleft()
             r.admitted = time.perf_counter()
-            r.opt = hf._VHFOpt(r.mol).build()
-            if r.kind == 'scf':
-                kw = dict(self.scf_defaults)
-                kw.update(r.scf_kwargs)
-                r.scf = RHF(r.mol, vhfopt=r.opt, **kw).build()
-            else:
-                r.dm = torch.as_tensor(r.dm, dtype=torch.float64, device=hf._device())
-            self.in_use += r.footprint
-            self.active.append(r)
-        self.stats['max_active'] = max(self.stats['max_active'], len(self.active))
-        self.stats['max_in_use'] = max(self.stats['max_in_use'], self.in_use)
+            li = bisect.bisect_right(self.lane_bounds, r.mol.nao_nr())
+            lane = self.lanes[li]
+            if lane.stream is None:
+                lane.stream = torch.cuda.Stream()
+            # set-up runs on a stream of its own, so it does not queue behind
+            # a build in flight in the request's lane
+            if self._setup_stream is None:
+                self._setup_stream = torch.cuda.Stream()
+            with hf.on_stream(self._setup_stream):
+                r.opt = hf._VHFOpt(r.mol).build()
+                if r.kind == 'scf':
+                    kw = dict(self.scf_defaults)
+                    kw.update(r.scf_kwargs)
+                    r.scf = RHF(r.mol, vhfopt=r.opt, stream=self._setup_stream, **kw).build()
+                    r.scf.stream = lane.stream
+                else:
+                    r.dm = torch.as_tensor(r.dm, dtype=torch.float64, device=hf._device())
+            self._setup_stream.synchronize()
+            with self._cv:
+                self.in_use += r.footprint
+                self.active.append(r)
+                lane.reqs.append(r)
+                self.stats['max_active'] = max(self.stats['max_active'], len(self.active))
+                self.stats['max_in_use'] = max(self.stats['max_in_use'], self.in_use)
+                if lane.thread is None:
+                    lane.thread = threading.Thread(target=self._lane_loop, args=(li,),
+                                                   daemon=True)
+                    lane.thread.start()
+                self._cv.notify_all()
 
     def _retire(self, r):
+        """Record a finished request's result (lane thread, on its stream)."""
         r.finished = time.perf_counter()
-        self.in_use -= r.footprint
         if r.opt is not None:
             r.opt.clear()
         r.opt = None
@@ -157,30 +219,30 @@ class Server:
                         'jk_builds': dict(s.jk_builds), 'dm': s.dm.cpu().numpy(),
                         'mo_energy': s.mo_energy.cpu().numpy()}
             r.scf = None
-        self.completed.append(r)
         if self.log:
             self.log(f'request {r.rid} ({r.kind}, {r.mol.nao_nr()} AOs) done in '
                      f'{r.latency:.2f} s, {r.jk_builds} J/K builds')
 
-    # -- one iteration -------------------------------------------------------------
-    def step(self):
-        """Admit, build J/K for every request in flight, advance each one.
-        Returns the requests finished in this iteration."""
-        self._admit()
-        if not self.active:
-            return []
+    # -- one lane --------------------------------------------------------------------
+    def _build(self, lane, reqs):
+        """Queue the batched J/K builds of reqs on the lane's stream."""
         groups = collections.defaultdict(list)
-        for r in self.active:
+        for r in reqs:
             prec = r.scf.next_precision() if r.kind == 'scf' else 'fp64'
             groups[(prec, r.opt.default_nbins())].append(r)
-        finished = []
+        builds = []
         for (prec, nbins), rs in groups.items():
             dms = [r.scf.dm if r.kind == 'scf' else r.dm for r in rs]
-            out = BatchJK(None, vhfopts=[r.opt for r in rs], precision=prec, nbins=nbins,
-                          fp32_cut=self.fp32_cut).build(dms)
-            self.stats['batched_builds'] += 1
-            self.stats['molecule_builds'] += len(rs)
-            self.stats['batch_sizes'].append(len(rs))
+            bj = BatchJK(None, vhfopts=[r.opt for r in rs], precision=prec, nbins=nbins,
+                         fp32_cut=self.fp32_cut, stream=lane.stream)
+            builds.append((prec, rs, bj, bj.build(dms)))
+        return builds
+
+    def _advance(self, builds):
+        """SCF step of every request of finished builds; returns the
+        requests that are done."""
+        finished = []
+        for prec, rs, _, out in builds:
             for r, (j, k) in zip(rs, out):
                 r.jk_builds += 1
                 if r.kind == 'jk':
@@ -188,19 +250,87 @@ class Server:
                     finished.append(r)
                 elif r.scf.absorb(j, k, prec) or r.scf.cycles >= r.scf.max_cycle:
                     finished.append(r)
-        self.stats['iterations'] += 1
-        for r in finished:
-            self.active.remove(r)
-            self._retire(r)
         return finished
 
+    def _lane_loop(self, li):
+        lane = self.lanes[li]
+        sec = self.stats['seconds']
+        try:
+            with hf.on_stream(lane.stream):
+                while True:
+                    with self._cv:
+                        while not lane.reqs and not self._closed:
+                            self._cv.wait()
+                        if not lane.reqs:
+                            return
+                        reqs = list(lane.reqs)
+                    t0 = time.perf_counter()
+                    builds = self._build(lane, reqs)
+                    t1 = time.perf_counter()
+                    lane.stream.synchronize()
+                    t2 = time.perf_counter()
+                    finished = self._advance(builds)
+                    for r in finished:
+                        self._retire(r)
+                    t3 = time.perf_counter()
+                    with self._cv:
+                        sec['launch'] += t1 - t0
+                        sec['wait'] += t2 - t1
+                        sec['scf'] += t3 - t2
+                        self.stats['iterations'] += 1
+                        self.stats['lane_builds'][li] += 1
+                        self.stats['batched_builds'] += len(builds)
+                        self.stats['molecule_builds'] += len(reqs)
+                        self.stats['batch_sizes'].extend(len(b[1]) for b in builds)
+                        for r in finished:
+                            lane.reqs.remove(r)
+                            self.active.remove(r)
+                            self.in_use -= r.footprint
+                            self.completed.append(r)
+                            self._done.append(r)
+                        self._cv.notify_all()
+        except BaseException as exc:   # surfaced by step()
+            with self._cv:
+                self._error = exc
+                self._cv.notify_all()
+
+    # -- the caller's loop ---------------------------------------------------------
+    def step(self, timeout=0.05):
+        """Admit what fits, then wait until a request finishes (at most
+        ``timeout`` seconds while requests are in flight).  Returns the
+        requests finished since the previous call."""
+        t0 = time.perf_counter()
+        self._admit()
+        self.stats['seconds']['admit'] += time.perf_counter() - t0
+        with self._cv:
+            if not self._done and self.active and self._error is None:
+                self._cv.wait(timeout)
+            if self._error is not None:
+                raise RuntimeError('a serving lane failed') from self._error
+            done, self._done = self._done, []
+        return done
+
     def idle(self):
-        return not self.queue and not self.active
+        with self._cv:
+            return not self.queue and not self.active
 
     def drain(self):
         while not self.idle():
             self.step()
         return self.completed
+
+    def close(self):
+        """Finish the requests in flight and stop the lane threads."""
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        for lane in self.lanes:
+            if lane.thread is not None:
+                lane.thread.join()
+                lane.thread = None
+        with self._cv:
+            if self._error is not None:
+                raise RuntimeError('a serving lane failed') from self._error
 
 
 def poisson_workload(qps, duration, seed, lo=5, hi=120, basis='def2-svp'):

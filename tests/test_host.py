@@ -208,3 +208,17 @@ def test_poisson_workload_and_footprint_estimate():
           for k in (1, 2, 4, 8)]
     assert all(b > a_ for a_, b in zip(fp, fp[1:]))
     assert fp[3] > 3 * fp[1]
+
+
+def test_fractional_aufbau_occupations():
+    """Spin-summed aufbau with a shared Fermi-level set: O (8 electrons) has
+    1s2 2s2 and 4/3 in each 2p; H has 1 in 1s; degenerate sets below the
+    Fermi level are full."""
+    from byteqc_amd.scf.rhf import _frac_occ
+    e = numpy.array([0.5, -20.0, -1.2, -0.6, -0.6 + 1e-9, -0.6 - 1e-9, 0.9])
+    occ = _frac_occ(e, 8)
+    assert numpy.allclose(occ, [0, 2, 2, 4 / 3, 4 / 3, 4 / 3, 0])
+    assert numpy.allclose(_frac_occ(numpy.array([-0.5, 0.2]), 1), [1, 0])
+    occ = _frac_occ(e, 10)
+    assert numpy.allclose(occ, [0, 2, 2, 2, 2, 2, 0]) and occ.sum() == 10
+    assert numpy.allclose(_frac_occ(e, 11).sum(), 11)

@@ -163,6 +163,27 @@ def test_rhf_known_answer_and_mixed_schedule():
     emx, mf2 = rhf.scf_energy(mol, conv_tol=1e-10, precision='mixed_then_fp64')
     print(f'RHF mixed->fp64 {emx:.10f} builds {mf2.jk_builds}', flush=True)
     assert mf2.converged and abs(emx - e64) < 1e-8
+    eh, mf3 = rhf.scf_energy(mol, conv_tol=1e-10, init_guess='hcore')
+    print(f'RHF core guess {eh:.10f} in {mf3.cycles} cycles', flush=True)
+    assert mf3.converged and abs(eh - e64) < 1e-9 and mf.cycles < mf3.cycles
+
+
+def test_atomic_densities_hold_the_atoms_electrons():
+    """Spherically averaged atomic densities: Tr(D S) is the electron count,
+    D is symmetric and its natural occupations lie in [0, 2]."""
+    _torch_gpu()
+    from byteqc_amd.gto import M
+    from byteqc_amd.scf import hf, rhf
+    for sym, basis, ne in (('O', 'cc-pvdz', 8), ('H', 'def2-svp', 1), ('O', 'cc-pvtz', 8)):
+        dm = rhf.atomic_density(sym, basis)
+        atom = M(atom=f'{sym} 0 0 0', basis=basis)
+        s = rhf.int1e(hf._VHFOpt(atom).build())[0].cpu().numpy()
+        assert abs(numpy.trace(dm @ s) - ne) < 1e-10
+        assert numpy.abs(dm - dm.T).max() < 1e-12
+        w, u = numpy.linalg.eigh(s)
+        half = u @ numpy.diag(numpy.sqrt(w)) @ u.T
+        occ = numpy.linalg.eigvalsh(half @ dm @ half)
+        assert occ.min() > -1e-10 and occ.max() < 2 + 1e-10
 
 
 def test_fp32_everywhere_cc_pvtz_stays_at_fp32_accuracy():

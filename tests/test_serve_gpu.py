@@ -107,3 +107,44 @@ def test_poisson_load_completes_with_latencies():
     assert len(reqs) == len(arr) and all(r.result['converged'] for r in reqs)
     lat = [r.latency for r in reqs]
     assert min(lat) > 0 and wall >= arr[-1][0]
+
+
+def test_size_lanes_run_concurrently_and_keep_results():
+    """Two lanes (split at 60 AOs): a small request iterates in its own lane
+    while a larger one builds; both give their standalone results."""
+    _torch_gpu()
+    from byteqc_amd.scf.serve import Server
+    from byteqc_amd.systems import water_mole, box_for_density
+    big = water_mole(6, box_for_density(6), 7, 'def2-svp')       # 144 AOs
+    small = _mols()[2]                                            # 24 AOs
+    ref = {}
+    for name, m in (('big', big), ('small', small)):
+        alone = Server(lanes=(60,))
+        r = alone.submit(m)
+        alone.drain()
+        ref[name] = r.result
+    srv = Server(lanes=(60,))
+    rb = srv.submit(big)
+    rs = srv.submit(small)
+    srv.drain()
+    assert srv.stats['lane_builds'][0] > 0 and srv.stats['lane_builds'][1] > 0
+    for name, r in (('big', rb), ('small', rs)):
+        assert r.error is None and r.result['converged']
+        assert abs(r.result['e_tot'] - ref[name]['e_tot']) <= 1e-9
+        assert numpy.abs(r.result['dm'] - ref[name]['dm']).max() <= 1e-7
+    print(f'lanes: builds {srv.stats["lane_builds"]}, latency small {rs.latency:.2f} s '
+          f'big {rb.latency:.2f} s', flush=True)
+    srv.close()
+    assert all(lane.thread is None for lane in srv.lanes)
+
+
+def test_lane_error_reaches_the_caller():
+    """An exception in a lane's host thread (here a density of the wrong
+    shape, rejected before any launch) is raised by the caller's step()."""
+    _torch_gpu()
+    from byteqc_amd.scf.serve import Server
+    mol = _mols()[2]
+    srv = Server()
+    srv.submit(mol, kind='jk', dm=numpy.zeros((mol.nao_nr() + 1, mol.nao_nr() + 1)))
+    with pytest.raises(RuntimeError, match='lane failed'):
+        srv.drain()
