@@ -14,6 +14,9 @@
 //        K(b,c) += g D(a,d)  lane registers over a run of equal c
 //        J(c,d), K(a,d), K(b,d): one FP64 atomic per element and quartet.
 // LDS per workgroup: 3*NR*GSZ + NF values of type T plus the density blocks.
+// The workgroup is one wavefront, so the phases are ordered with wavefront
+// fences (LDS operations of one wave complete in order) instead of
+// workgroup barriers.
 #pragma once
 #include "jk_v2.h"
 
@@ -25,6 +28,7 @@ struct WaveCls {
     static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NAD = NA * ND,
                          NBC = NB * NC, NBD = NB * ND;
     static constexpr int EAB = (NAB + 63) / 64, EAC = (NAC + 63) / 64, EBC = (NBC + 63) / 64;
+    static constexpr int ECD = (NCD + 63) / 64, EBD = (NBD + 63) / 64, EAD = (NAD + 63) / 64;
 };
 
 // I-table index of component n = ((x*NB + y)*NC + z)*ND + w for direction d
@@ -36,6 +40,19 @@ __device__ __forceinline__ int gidx(int n, int d) {
               x = n / (W::ND * W::NC * W::NB);
     auto e = [d](int l, int c) { return d == 0 ? cart_x(l, c) : (d == 1 ? cart_y(l, c) : cart_z(l, c)); };
     return C::g(e(LA, x), e(LB, y), e(LC, z), e(LD, w));
+}
+
+// Whether the kernel loads a quartet's density elements before its integral
+// phases (hiding their latency behind phases A and B) or in the digestion
+// phase.  The early loads hold registers through phases A and B; in the
+// classes listed here that cost one resident wave per SIMD and made the
+// kernel 13-43 % slower on MI355X, so they load late.  Elsewhere early loads
+// gain up to 14 % (measured per class on 128 H2O cc-pVTZ).
+template <int LA, int LB, int LC, int LD>
+constexpr bool wave_prefetch() {
+    constexpr int c = LA * 1000 + LB * 100 + LC * 10 + LD;
+    return c != 3021 && c != 3120 && c != 3211 && c != 3220 && c != 3221 && c != 3231 &&
+           c != 3232 && c != 3321 && c != 3322 && c != 3331 && c != 3332;
 }
 
 // Work item: (task, P position, first and end Q position).
@@ -78,13 +95,16 @@ __global__ void __launch_bounds__(64)
     const double ra[3] = {ubx[sa], uby[sa], ubz[sa]};
     const double rb[3] = {ubx[sb], uby[sb], ubz[sb]};
     const double ab[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-    // I-table indices of this lane's components
-    int gi[W::NG][3];
+    // I-table indices of this lane's components, x | y << 10 | z << 20 (one
+    // register per component instead of three)
+    static_assert(C::GSZ <= 1024, "I-table index does not fit 10 bits");
+    int gi[W::NG];
 #pragma unroll
     for (int m = 0; m < W::NG; ++m) {
         const int n = lane + 64 * m;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gi[m][d] = n < C::NF ? gidx<LA, LB, LC, LD>(n, d) : 0;
+        gi[m] = n < C::NF ? gidx<LA, LB, LC, LD>(n, 0) | gidx<LA, LB, LC, LD>(n, 1) << 10 |
+                                gidx<LA, LB, LC, LD>(n, 2) << 20
+                          : 0;
     }
     const int nao = jk.nao;
     const size_t nao2 = (size_t)nao * nao;
@@ -108,6 +128,36 @@ __global__ void __launch_bounds__(64)
             const int sd = uket[gq];
             const int c0 = uao[sc], d0 = uao[sd];
             const bool skip = tk.diag && ppos < qpos;
+            // density elements of this Q pair, stored to LDS for the digestion
+            double pcd[W::ECD], pbd[W::EBD], pad[W::EAD], pbc[W::EBC], pac[W::EAC];
+            constexpr bool PF = wave_prefetch<LA, LB, LC, LD>();
+            if constexpr (PF) {  // loads overlap phases A and B
+#pragma unroll
+                for (int m = 0; m < W::ECD; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NCD) pcd[m] = D[(size_t)(c0 + e / ND) * nao + d0 + e % ND];
+                }
+#pragma unroll
+                for (int m = 0; m < W::EBD; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NBD) pbd[m] = D[(size_t)(d0 + e % ND) * nao + b0 + e / ND];
+                }
+#pragma unroll
+                for (int m = 0; m < W::EAD; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NAD) pad[m] = D[(size_t)(d0 + e % ND) * nao + a0 + e / ND];
+                }
+#pragma unroll
+                for (int m = 0; m < W::EBC; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NBC) pbc[m] = D[(size_t)(c0 + e % NC) * nao + b0 + e / NC];
+                }
+#pragma unroll
+                for (int m = 0; m < W::EAC; ++m) {
+                    const int e = lane + 64 * m;
+                    if (e < W::NAC) pac[m] = D[(size_t)(c0 + e % NC) * nao + a0 + e / NC];
+                }
+            }
             double norm = TWO_PI_52;
             if (tk.diag && ppos == qpos) norm *= 0.5;
             T g[W::NG];
@@ -160,22 +210,23 @@ __global__ void __launch_bounds__(64)
                                                      (T)((Qd - Cd) + 2.0 * b00 * aij * pqd), (T)b10,
                                                      (T)b01, (T)b00, (T)abd, (T)cdd);
                         }
-                        __syncthreads();
+                        wave_sync();
                         // phase B: components
 #pragma unroll
                         for (int m = 0; m < W::NG; ++m) {
                             if (lane + 64 * m < C::NF) {
                                 T acc = (T)0;
+                                const int ix = gi[m] & 1023, iy = (gi[m] >> 10) & 1023,
+                                          iz = gi[m] >> 20;
 #pragma unroll
                                 for (int r = 0; r < NR; ++r) {
                                     const T *I0 = sI + (r * 3) * C::GSZ;
-                                    acc += I0[gi[m][0]] * I0[C::GSZ + gi[m][1]] *
-                                           I0[2 * C::GSZ + gi[m][2]];
+                                    acc += I0[ix] * I0[C::GSZ + iy] * I0[2 * C::GSZ + iz];
                                 }
                                 g[m] += acc;
                             }
                         }
-                        __syncthreads();
+                        wave_sync();
                     }
                 }
             }
@@ -183,17 +234,35 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int m = 0; m < W::NG; ++m)
                 if (lane + 64 * m < C::NF) sg[lane + 64 * m] = g[m];
-            for (int e = lane; e < W::NCD; e += 64)
-                sDcd[e] = D[(size_t)(c0 + e / ND) * nao + d0 + e % ND];
-            for (int e = lane; e < W::NBD; e += 64)
-                sDbd[e] = D[(size_t)(d0 + e % ND) * nao + b0 + e / ND];
-            for (int e = lane; e < W::NAD; e += 64)
-                sDad[e] = D[(size_t)(d0 + e % ND) * nao + a0 + e / ND];
-            for (int e = lane; e < W::NBC; e += 64)
-                sDbc[e] = D[(size_t)(c0 + e % NC) * nao + b0 + e / NC];
-            for (int e = lane; e < W::NAC; e += 64)
-                sDac[e] = D[(size_t)(c0 + e % NC) * nao + a0 + e / NC];
-            __syncthreads();
+            if constexpr (PF) {
+#pragma unroll
+                for (int m = 0; m < W::ECD; ++m)
+                    if (lane + 64 * m < W::NCD) sDcd[lane + 64 * m] = pcd[m];
+#pragma unroll
+                for (int m = 0; m < W::EBD; ++m)
+                    if (lane + 64 * m < W::NBD) sDbd[lane + 64 * m] = pbd[m];
+#pragma unroll
+                for (int m = 0; m < W::EAD; ++m)
+                    if (lane + 64 * m < W::NAD) sDad[lane + 64 * m] = pad[m];
+#pragma unroll
+                for (int m = 0; m < W::EBC; ++m)
+                    if (lane + 64 * m < W::NBC) sDbc[lane + 64 * m] = pbc[m];
+#pragma unroll
+                for (int m = 0; m < W::EAC; ++m)
+                    if (lane + 64 * m < W::NAC) sDac[lane + 64 * m] = pac[m];
+            } else {
+                for (int e = lane; e < W::NCD; e += 64)
+                    sDcd[e] = D[(size_t)(c0 + e / ND) * nao + d0 + e % ND];
+                for (int e = lane; e < W::NBD; e += 64)
+                    sDbd[e] = D[(size_t)(d0 + e % ND) * nao + b0 + e / ND];
+                for (int e = lane; e < W::NAD; e += 64)
+                    sDad[e] = D[(size_t)(d0 + e % ND) * nao + a0 + e / ND];
+                for (int e = lane; e < W::NBC; e += 64)
+                    sDbc[e] = D[(size_t)(c0 + e % NC) * nao + b0 + e / NC];
+                for (int e = lane; e < W::NAC; e += 64)
+                    sDac[e] = D[(size_t)(c0 + e % NC) * nao + a0 + e / NC];
+            }
+            wave_sync();
 #define SG(x, y, z, w) ((double)sg[(((x) * NB + (y)) * NC + (z)) * ND + (w)])
             if (!skip) {
                 if (VJ) {
@@ -256,7 +325,7 @@ __global__ void __launch_bounds__(64)
                 }
             }
 #undef SG
-            __syncthreads();
+            wave_sync();
             // end of a run of equal c: flush K(a,c), K(b,c)
             const int sc_next = q + 1 < q1 ? ubra[tk.bas_q0 + uperm_q[q + 1]] : -1;
             if (VK && sc_next != sc) {
@@ -285,7 +354,7 @@ __global__ void __launch_bounds__(64)
                     atomic_add(VJ + (size_t)(a0 + e / NB) * nao + b0 + e % NB, jab[m]);
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
