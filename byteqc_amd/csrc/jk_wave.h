@@ -8,7 +8,8 @@
 //   B. the 64 lanes split the Cartesian components and accumulate the
 //      contracted block over roots and primitive quartets in registers;
 //   C. the block goes to LDS and the lanes split the elements of the six
-//      digestion targets:
+//      digestion targets, several lanes per element where the sums are long
+//      (digest_split.h):
 //        J(a,b) += g D(c,d)  lane registers, flushed once per item
 //        K(a,c) += g D(b,d)  lane registers over a run of equal c
 //        K(b,c) += g D(a,d)  lane registers over a run of equal c
@@ -18,6 +19,7 @@
 // fences (LDS operations of one wave complete in order) instead of
 // workgroup barriers.
 #pragma once
+#include "digest_split.h"
 #include "jk_v2.h"
 
 template <int LA, int LB, int LC, int LD>
@@ -27,9 +29,37 @@ struct WaveCls {
     static constexpr int NG = (C::NF + 63) / 64;              // block values per lane
     static constexpr int NAB = NA * NB, NCD = NC * ND, NAC = NA * NC, NAD = NA * ND,
                          NBC = NB * NC, NBD = NB * ND;
-    static constexpr int EAB = (NAB + 63) / 64, EAC = (NAC + 63) / 64, EBC = (NBC + 63) / 64;
-    static constexpr int ECD = (NCD + 63) / 64, EBD = (NBD + 63) / 64, EAD = (NAD + 63) / 64;
+    // density elements per lane of each block
+    static constexpr int EAC = (NAC + 63) / 64, EBC = (NBC + 63) / 64, ECD = (NCD + 63) / 64,
+                         EBD = (NBD + 63) / 64, EAD = (NAD + 63) / 64;
 };
+
+// Sum of digestion target TG for element e over the outer indices u = s,
+// s + G, ... of lane s of its group, added over the group's G lanes (the
+// group's lanes all hold the total).  Every lane of the wavefront calls it;
+// lanes of groups past the last element pass valid = false.
+template <class TG, typename T>
+__device__ __forceinline__ double split_sum(const T *sg, const double *sd, int e, int s,
+                                            bool valid) {
+    using S = Split<TG>;
+    double sum = 0.0;
+    if (valid) {
+        const T *gp = sg + TG::gbase(e) + s * TG::SU;
+        const double *dp = sd + s * TG::DU;
+#pragma unroll
+        for (int i = 0; i < S::IU; ++i) {
+            if (TG::NU % S::G == 0 || s + S::G * i < TG::NU) {
+#pragma unroll
+                for (int v = 0; v < TG::NV; ++v)
+                    sum += (double)gp[i * S::G * TG::SU + v * TG::SV] *
+                           dp[i * S::G * TG::DU + v * TG::DV];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S::G; o <<= 1) sum += __shfl_xor(sum, o, 64);
+    return sum;
+}
 
 // I-table index of component n = ((x*NB + y)*NC + z)*ND + w for direction d
 template <int LA, int LB, int LC, int LD>
@@ -62,6 +92,13 @@ __global__ void __launch_bounds__(64)
     using W = WaveCls<LA, LB, LC, LD>;
     using C = typename W::C;
     constexpr int NR = C::NR, NA = W::NA, NB = W::NB, NC = W::NC, ND = W::ND;
+    using DT = DigestTargets<NA, NB, NC, ND>;
+    using SAB = Split<typename DT::Jab>;
+    using SCD = Split<typename DT::Jcd>;
+    using SAC = Split<typename DT::Kac>;
+    using SBC = Split<typename DT::Kbc>;
+    using SAD = Split<typename DT::Kad>;
+    using SBD = Split<typename DT::Kbd>;
     __shared__ T sI[3 * NR * C::GSZ];
     __shared__ T sg[C::NF];
     __shared__ double sD[W::NAB + W::NCD + W::NBD + W::NAD + W::NBC + W::NAC];
@@ -114,13 +151,15 @@ __global__ void __launch_bounds__(64)
         gdouble *VK = jk.vk ? glb_rw(jk.vk + idm * nao2) : nullptr;
         for (int e = lane; e < W::NAB; e += 64)
             sDab[e] = D[(size_t)(a0 + e / NB) * nao + b0 + e % NB];
-        double jab[W::EAB], kac[W::EAC], kbc[W::EBC];
+        // register targets: element p * PER + lane / G, held by the group's
+        // first lane (the others keep zeros)
+        double jab[SAB::PASSES], kac[SAC::PASSES], kbc[SBC::PASSES];
 #pragma unroll
-        for (int m = 0; m < W::EAB; ++m) jab[m] = 0.0;
+        for (int m = 0; m < SAB::PASSES; ++m) jab[m] = 0.0;
 #pragma unroll
-        for (int m = 0; m < W::EAC; ++m) kac[m] = 0.0;
+        for (int m = 0; m < SAC::PASSES; ++m) kac[m] = 0.0;
 #pragma unroll
-        for (int m = 0; m < W::EBC; ++m) kbc[m] = 0.0;
+        for (int m = 0; m < SBC::PASSES; ++m) kbc[m] = 0.0;
         int sc = ubra[tk.bas_q0 + uperm_q[q0]];
         for (int q = q0; q < q1; ++q) {
             const int qpos = uperm_q[q];
@@ -263,83 +302,66 @@ __global__ void __launch_bounds__(64)
                     sDac[e] = D[(size_t)(c0 + e % NC) * nao + a0 + e / NC];
             }
             wave_sync();
-#define SG(x, y, z, w) ((double)sg[(((x) * NB + (y)) * NC + (z)) * ND + (w)])
             if (!skip) {
                 if (VJ) {
 #pragma unroll
-                    for (int m = 0; m < W::EAB; ++m) {
-                        const int e = lane + 64 * m;
-                        if (e < W::NAB) {
-                            const int x = e / NB, y = e % NB;
-                            double sum = 0.0;
-                            for (int z = 0; z < NC; ++z)
-                                for (int w = 0; w < ND; ++w) sum += SG(x, y, z, w) * sDcd[z * ND + w];
-                            jab[m] += sum;
-                        }
+                    for (int m = 0; m < SAB::PASSES; ++m) {
+                        const int e = m * SAB::PER + lane / SAB::G, sl = lane % SAB::G;
+                        const double v = split_sum<typename DT::Jab>(sg, sDcd, e, sl, e < W::NAB);
+                        if (sl == 0) jab[m] += v;
                     }
-                    for (int e = lane; e < W::NCD; e += 64) {
-                        const int z = e / ND, w = e % ND;
-                        double sum = 0.0;
-                        for (int x = 0; x < NA; ++x)
-                            for (int y = 0; y < NB; ++y) sum += SG(x, y, z, w) * sDab[x * NB + y];
-                        atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, sum);
+#pragma unroll
+                    for (int m = 0; m < SCD::PASSES; ++m) {
+                        const int e = m * SCD::PER + lane / SCD::G, sl = lane % SCD::G;
+                        const double v = split_sum<typename DT::Jcd>(sg, sDab, e, sl, e < W::NCD);
+                        if (sl == 0 && e < W::NCD)
+                            atomic_add(VJ + (size_t)(c0 + e / ND) * nao + d0 + e % ND, v);
                     }
                 }
                 if (VK) {
 #pragma unroll
-                    for (int m = 0; m < W::EAC; ++m) {
-                        const int e = lane + 64 * m;
-                        if (e < W::NAC) {
-                            const int x = e / NC, z = e % NC;
-                            double sum = 0.0;
-                            for (int y = 0; y < NB; ++y)
-                                for (int w = 0; w < ND; ++w) sum += SG(x, y, z, w) * sDbd[y * ND + w];
-                            kac[m] += sum;
-                        }
+                    for (int m = 0; m < SAC::PASSES; ++m) {
+                        const int e = m * SAC::PER + lane / SAC::G, sl = lane % SAC::G;
+                        const double v = split_sum<typename DT::Kac>(sg, sDbd, e, sl, e < W::NAC);
+                        if (sl == 0) kac[m] += v;
                     }
 #pragma unroll
-                    for (int m = 0; m < W::EBC; ++m) {
-                        const int e = lane + 64 * m;
-                        if (e < W::NBC) {
-                            const int y = e / NC, z = e % NC;
-                            double sum = 0.0;
-                            for (int x = 0; x < NA; ++x)
-                                for (int w = 0; w < ND; ++w) sum += SG(x, y, z, w) * sDad[x * ND + w];
-                            kbc[m] += sum;
-                        }
+                    for (int m = 0; m < SBC::PASSES; ++m) {
+                        const int e = m * SBC::PER + lane / SBC::G, sl = lane % SBC::G;
+                        const double v = split_sum<typename DT::Kbc>(sg, sDad, e, sl, e < W::NBC);
+                        if (sl == 0) kbc[m] += v;
                     }
-                    for (int e = lane; e < W::NAD; e += 64) {
-                        const int x = e / ND, w = e % ND;
-                        double sum = 0.0;
-                        for (int y = 0; y < NB; ++y)
-                            for (int z = 0; z < NC; ++z) sum += SG(x, y, z, w) * sDbc[y * NC + z];
-                        atomic_add(VK + (size_t)(d0 + w) * nao + a0 + x, sum);
+#pragma unroll
+                    for (int m = 0; m < SAD::PASSES; ++m) {
+                        const int e = m * SAD::PER + lane / SAD::G, sl = lane % SAD::G;
+                        const double v = split_sum<typename DT::Kad>(sg, sDbc, e, sl, e < W::NAD);
+                        if (sl == 0 && e < W::NAD)
+                            atomic_add(VK + (size_t)(d0 + e % ND) * nao + a0 + e / ND, v);
                     }
-                    for (int e = lane; e < W::NBD; e += 64) {
-                        const int y = e / ND, w = e % ND;
-                        double sum = 0.0;
-                        for (int x = 0; x < NA; ++x)
-                            for (int z = 0; z < NC; ++z) sum += SG(x, y, z, w) * sDac[x * NC + z];
-                        atomic_add(VK + (size_t)(d0 + w) * nao + b0 + y, sum);
+#pragma unroll
+                    for (int m = 0; m < SBD::PASSES; ++m) {
+                        const int e = m * SBD::PER + lane / SBD::G, sl = lane % SBD::G;
+                        const double v = split_sum<typename DT::Kbd>(sg, sDac, e, sl, e < W::NBD);
+                        if (sl == 0 && e < W::NBD)
+                            atomic_add(VK + (size_t)(d0 + e % ND) * nao + b0 + e / ND, v);
                     }
                 }
             }
-#undef SG
             wave_sync();
             // end of a run of equal c: flush K(a,c), K(b,c)
             const int sc_next = q + 1 < q1 ? ubra[tk.bas_q0 + uperm_q[q + 1]] : -1;
             if (VK && sc_next != sc) {
 #pragma unroll
-                for (int m = 0; m < W::EAC; ++m) {
-                    const int e = lane + 64 * m;
-                    if (e < W::NAC && kac[m] != 0.0)
+                for (int m = 0; m < SAC::PASSES; ++m) {
+                    const int e = m * SAC::PER + lane / SAC::G;
+                    if (lane % SAC::G == 0 && e < W::NAC && kac[m] != 0.0)
                         atomic_add(VK + (size_t)(c0 + e % NC) * nao + a0 + e / NC, kac[m]);
                     kac[m] = 0.0;
                 }
 #pragma unroll
-                for (int m = 0; m < W::EBC; ++m) {
-                    const int e = lane + 64 * m;
-                    if (e < W::NBC && kbc[m] != 0.0)
+                for (int m = 0; m < SBC::PASSES; ++m) {
+                    const int e = m * SBC::PER + lane / SBC::G;
+                    if (lane % SBC::G == 0 && e < W::NBC && kbc[m] != 0.0)
                         atomic_add(VK + (size_t)(c0 + e % NC) * nao + b0 + e / NC, kbc[m]);
                     kbc[m] = 0.0;
                 }
@@ -348,9 +370,9 @@ __global__ void __launch_bounds__(64)
         }
         if (VJ) {
 #pragma unroll
-            for (int m = 0; m < W::EAB; ++m) {
-                const int e = lane + 64 * m;
-                if (e < W::NAB && jab[m] != 0.0)
+            for (int m = 0; m < SAB::PASSES; ++m) {
+                const int e = m * SAB::PER + lane / SAB::G;
+                if (lane % SAB::G == 0 && e < W::NAB && jab[m] != 0.0)
                     atomic_add(VJ + (size_t)(a0 + e / NB) * nao + b0 + e % NB, jab[m]);
             }
         }
