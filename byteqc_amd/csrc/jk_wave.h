@@ -75,14 +75,13 @@ __device__ __forceinline__ int gidx(int n, int d) {
 // Whether the kernel loads a quartet's density elements before its integral
 // phases (hiding their latency behind phases A and B) or in the digestion
 // phase.  The early loads hold registers through phases A and B; in the
-// classes listed here that cost one resident wave per SIMD and made the
-// kernel 13-43 % slower on MI355X, so they load late.  Elsewhere early loads
-// gain up to 14 % (measured per class on 128 H2O cc-pVTZ).
+// classes listed here that cost a resident wave per SIMD and made the
+// kernel slower on MI355X (by 18-22 %), so they load late.  Elsewhere early
+// loads gain up to 14 % (measured per class on 128 H2O cc-pVTZ).
 template <int LA, int LB, int LC, int LD>
 constexpr bool wave_prefetch() {
     constexpr int c = LA * 1000 + LB * 100 + LC * 10 + LD;
-    return c != 3021 && c != 3120 && c != 3211 && c != 3220 && c != 3221 && c != 3231 &&
-           c != 3232 && c != 3321 && c != 3322 && c != 3331 && c != 3332;
+    return c != 2211 && c != 3310;
 }
 
 // Work item: (task, P position, first and end Q position).
@@ -199,9 +198,19 @@ __global__ void __launch_bounds__(64)
             }
             double norm = TWO_PI_52;
             if (tk.diag && ppos == qpos) norm *= 0.5;
-            T g[W::NG];
+            // the block accumulates in registers, or for large blocks (more than
+            // 16 values per lane) in this lane's own elements of sg, which
+            // keeps the registers for the rest and avoids spills
+            constexpr bool GL = W::NG > 16;
+            T g[GL ? 1 : W::NG];
 #pragma unroll
-            for (int m = 0; m < W::NG; ++m) g[m] = (T)0;
+            for (int m = 0; m < W::NG; ++m) {
+                if constexpr (GL) {
+                    if (lane + 64 * m < C::NF) sg[lane + 64 * m] = (T)0;
+                } else {
+                    g[m] = (T)0;
+                }
+            }
             if (!skip) {
                 const double rc[3] = {ubx[sc], uby[sc], ubz[sc]};
                 const double rd[3] = {ubx[sd], uby[sd], ubz[sd]};
@@ -262,7 +271,10 @@ __global__ void __launch_bounds__(64)
                                     const T *I0 = sI + (r * 3) * C::GSZ;
                                     acc += I0[ix] * I0[C::GSZ + iy] * I0[2 * C::GSZ + iz];
                                 }
-                                g[m] += acc;
+                                if constexpr (GL)
+                                    sg[lane + 64 * m] += acc;
+                                else
+                                    g[m] += acc;
                             }
                         }
                         wave_sync();
@@ -270,9 +282,11 @@ __global__ void __launch_bounds__(64)
                 }
             }
             // phase C: digestion
+            if constexpr (!GL) {
 #pragma unroll
-            for (int m = 0; m < W::NG; ++m)
-                if (lane + 64 * m < C::NF) sg[lane + 64 * m] = g[m];
+                for (int m = 0; m < W::NG; ++m)
+                    if (lane + 64 * m < C::NF) sg[lane + 64 * m] = g[m];
+            }
             if constexpr (PF) {
 #pragma unroll
                 for (int m = 0; m < W::ECD; ++m)
