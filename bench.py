@@ -178,7 +178,7 @@ def run_single(args, torch):
     log(f'plan {plan_s:.2f} s')
     tuned = None
     if args.autotune:
-        table = autotune.tune(step)
+        table = autotune.tune(step, nao=int(opt.coeff.shape[0]))
         tuned = dict(zip(('default_ms', 'tuned_ms'), autotune.gain(table)))
         log(f'autotuned {len(table)} (kind, class, precision) tiles: {tuned}')
     ref = None
@@ -272,7 +272,7 @@ def run_batch(args, torch):
         # tiles are per class, so a sub-batch with the same size mix suffices
         sub = BatchJK(mols[:args.tune_nmol], vhfopts=opts[:args.tune_nmol],
                       precision=args.precision)
-        table = autotune.tune(lambda: sub.build(dms[:args.tune_nmol]))
+        table = autotune.tune(lambda: sub.build(dms[:args.tune_nmol]), nao=max(sub.nao_c))
         del sub
         tuned = dict(zip(('default_ms', 'tuned_ms'), autotune.gain(table)))
         tuned['tuned_on_molecules'] = args.tune_nmol

@@ -307,11 +307,29 @@ static int g_ck_rows = BQC_CK_ROWS_DEFAULT;  // Q pairs per row-kernel item
 static int g_ck_wave = BQC_CK_WAVE_DEFAULT;  // Q pairs per wave-kernel item
 static std::vector<std::pair<long long, int>> g_ck_override;  // (key, ck)
 
-static int tile_for(int kind, int cls, int prec) {
+// Default tiles by molecule size.  Autotuner timings on MI355X: for large
+// molecules (64 H2O cc-pVDZ, 2624 Cartesian AOs; 128 H2O cc-pVTZ) the row
+// kernels of the s/p classes are fastest with 256 Q pairs per item (fewer
+// flushes of the lane accumulators; 4-6 % over 64), the spilling 108-component
+// classes with 32-64 and the wave kernels with 64; for batches of small
+// molecules (4-40 H2O def2-SVP, at most ~1000 AOs) with 32, 16 and 16.
+#define BQC_TILE_BIG_NAO 2048
+static bool g_ck_all_set[2] = {false, false};  // a tile set for all classes of a kind
+
+static int class_components(int cls) {
+    auto nc = [](int l) { return (l + 1) * (l + 2) / 2; };
+    return nc(cls / 1000) * nc(cls / 100 % 10) * nc(cls / 10 % 10) * nc(cls % 10);
+}
+
+static int tile_for(int kind, int cls, int prec, int nao) {
     long long key = ((long long)kind * 10000 + cls) * 2 + prec;
     for (auto &kv : g_ck_override)
         if (kv.first == key) return kv.second;
-    return kind == 1 ? g_ck_rows : g_ck_wave;
+    if (g_ck_all_set[kind == 1 ? 0 : 1]) return kind == 1 ? g_ck_rows : g_ck_wave;
+    const bool big = nao >= BQC_TILE_BIG_NAO;
+    if (kind == 4) return big ? 64 : 16;
+    if (class_components(cls) > 81) return big ? 64 : 16;
+    return big ? 256 : 32;
 }
 
 struct Group {
@@ -456,7 +474,7 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
     // precisions cover each pair's Q range exactly once.
     const bool by_row = !wv[0] || !wv[1];
     for (int prec = 0; prec < 2; ++prec) {
-        const int ck = tile_for(wv[prec] ? 4 : 1, cls, prec);
+        const int ck = tile_for(wv[prec] ? 4 : 1, cls, prec, jk.nao);
         const int step = wv[prec] ? 1 : 64;
         std::vector<int> &it = g[prec]->items;
         for (int p = 0; p < np; p += step) {
@@ -783,12 +801,14 @@ int BQCset_tile(int kind, int cls, int prec, int ck) {
         g_ck_override.clear();
         g_ck_rows = BQC_CK_ROWS_DEFAULT;
         g_ck_wave = BQC_CK_WAVE_DEFAULT;
+        g_ck_all_set[0] = g_ck_all_set[1] = false;
         return 0;
     }
     if (ck < 1 || ck > 4096 || (kind != 1 && kind != 4)) return 1;
     if (cls < 0) {
         if (kind == 1) g_ck_rows = ck;
         else g_ck_wave = ck;
+        g_ck_all_set[kind == 1 ? 0 : 1] = true;
         return 0;
     }
     long long key = ((long long)kind * 10000 + cls) * 2 + prec;
@@ -799,6 +819,11 @@ int BQCset_tile(int kind, int cls, int prec, int ck) {
         }
     g_ck_override.push_back({key, ck});
     return 0;
+}
+
+int BQCget_tile(int kind, int cls, int prec, int nao) {
+    if (kind != 1 && kind != 4) return -1;
+    return tile_for(kind, cls, prec, nao);
 }
 
 void BQCprofile_enable(int on) {

@@ -40,6 +40,7 @@ _SIGNATURES = {
     'BQCgraph_launch': (c_int, [c_void_p, c_void_p]),
     'BQCgraph_destroy': (None, [c_void_p]),
     'BQCset_tile': (c_int, [c_int, c_int, c_int, c_int]),
+    'BQCget_tile': (c_int, [c_int, c_int, c_int, c_int]),
     'BQCplan_tasks': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double,
                               c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     'BQCdm_cond': (c_int, [c_void_p] * 7 + [c_int, c_int, c_int, c_void_p]),

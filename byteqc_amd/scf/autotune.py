@@ -11,7 +11,9 @@ a large ck gives fewer flushes and fewer items.
 ``tune(run)`` times every candidate with the library's per-launch HIP-event
 records (one launch per class, kind and precision per build), keeps the
 fastest ck per (kind, class, precision) and installs it with
-``BQCset_tile``.  Any tile gives the same quartet set and the same sums
+``BQCset_tile``.  Without tuning the library uses default tiles chosen by
+molecule size (large molecules want long Q ranges per item, batches of
+small ones short), measured the same way over the benchmark workloads.  Any tile gives the same quartet set and the same sums
 (only the order of floating-point accumulation differs).
 
 The reference has fixed launch shapes per class (nr_jk_driver.cu:46-225).
@@ -28,12 +30,18 @@ from byteqc_amd.scf import hf
 
 ROWS, WAVE = 1, 4
 CANDIDATES = {ROWS: (16, 32, 64, 128, 256), WAVE: (8, 16, 32, 64, 128)}
-DEFAULTS = {ROWS: 64, WAVE: 32}
+DEFAULTS = {ROWS: 64, WAVE: 32}     # reference point for a table without recorded defaults
 
 
 def reset():
-    """Drop every tuned tile and restore the defaults."""
+    """Drop every tuned tile and restore the library's default tiles."""
     load_library().BQCset_tile(0, -1, 0, 0)
+
+
+def current_tile(kind, cls, prec, nao):
+    """Tile a build of a molecule with nao Cartesian AOs planned now uses for
+    (kind, cls, prec): a tuned tile if one is installed, else the default."""
+    return int(load_library().BQCget_tile(int(kind), int(cls), int(prec), int(nao)))
 
 
 def _timed(run, reps):
@@ -50,10 +58,12 @@ def _timed(run, reps):
     return hf.profile_read()
 
 
-def tune(run, candidates=None, reps=1):
+def tune(run, candidates=None, reps=1, nao=0):
     """Tune tiles for the builds done by ``run()`` (one eager J/K build per
-    call, any number of molecules).  Returns {(kind, cls, prec): {'ck': best,
-    'ms': {ck: ms per build}}} and leaves the best tiles installed."""
+    call, any number of molecules; nao: Cartesian AO count of the largest,
+    which picks the default tiles recorded as 'builtin').  Returns
+    {(kind, cls, prec): {'ck': best, 'ms': {ck: ms per build}, 'builtin':
+    default ck}} and leaves the best tiles installed."""
     lib = load_library()
     candidates = candidates or CANDIDATES
     reset()
@@ -69,11 +79,11 @@ def tune(run, candidates=None, reps=1):
                 agg[(r['cls'], (r['kind'] >> 1) & 1)] += r['ms']
             for (cls, prec), ms in agg.items():
                 times[(kind, cls, prec)][int(ck)] = ms / reps
-        lib.BQCset_tile(kind, -1, 0, DEFAULTS[kind])
+        reset()
     table = {}
     for key, per in times.items():
         best = min(per, key=per.get)
-        table[key] = {'ck': best, 'ms': per}
+        table[key] = {'ck': best, 'ms': per, 'builtin': current_tile(*key, nao)}  # after reset()
     install(table)
     return table
 
@@ -88,15 +98,16 @@ def install(table):
 
 
 def gain(table):
-    """(ms per build with default tiles, ms with tuned tiles) over the table."""
-    base = sum(rec['ms'].get(DEFAULTS[k], min(rec['ms'].values()))
+    """(ms per build with the default tiles, ms with tuned tiles) over the
+    table."""
+    base = sum(rec['ms'].get(rec.get('builtin', DEFAULTS[k]), min(rec['ms'].values()))
                for (k, _, _), rec in table.items())
     tuned = sum(rec['ms'][rec['ck']] for rec in table.values())
     return base, tuned
 
 
 def save(table, path):
-    rows = [{'kind': k, 'cls': c, 'prec': p, 'ck': rec['ck'],
+    rows = [{'kind': k, 'cls': c, 'prec': p, 'ck': rec['ck'], 'builtin': rec.get('builtin'),
              'ms': {str(ck): ms for ck, ms in rec['ms'].items()}}
             for (k, c, p), rec in sorted(table.items())]
     with open(path, 'w') as f:
@@ -106,6 +117,11 @@ def save(table, path):
 def load(path):
     with open(path) as f:
         rows = json.load(f)
-    return {(r['kind'], r['cls'], r['prec']): {'ck': r['ck'],
-                                               'ms': {int(k): v for k, v in r['ms'].items()}}
-            for r in rows}
+    out = {}
+    for r in rows:
+        rec = {'ck': r['ck'], 'ms': {int(k): v for k, v in r['ms'].items()}}
+        if r.get('builtin') is not None:
+            rec['builtin'] = r['builtin']
+        out[(r['kind'], r['cls'], r['prec'])] = rec
+    return out
+

@@ -156,11 +156,19 @@ void BQCgraph_destroy(void *graph);
 /* Q pairs per work item of the row kernels (kind 1) or wave kernels
  * (kind 4), for class code cls (li*1000+lj*100+lk*10+ll as evaluated) and
  * precision prec (0 FP64, 1 FP32), or for all classes of the kind (cls < 0);
- * kind 0 clears all per-class settings and restores the defaults (64, 32).
- * Process-wide setting read when a build is planned; used by the tile
- * autotuner.  No counterpart in the reference (its launch shapes are fixed
- * per class, nr_jk_driver.cu:46-225).  Returns 0 ok, 1 bad argument. */
+ * kind 0 clears these settings and restores the default tiles, which depend
+ * on the molecule's Cartesian AO count (measured on MI355X: rows 256 / 64
+ * for classes above 81 components / waves 64 from 2048 AOs up, else
+ * 32 / 16 / 16).  Precedence when a build is planned: per-class setting,
+ * all-class setting, default.  Process-wide; used by the tile autotuner.
+ * No counterpart in the reference (its launch shapes are fixed per class,
+ * nr_jk_driver.cu:46-225).  Returns 0 ok, 1 bad argument. */
 int BQCset_tile(int kind, int cls, int prec, int ck);
+
+/* The tile a build of a molecule with nao Cartesian AOs planned now would
+ * use for (kind, cls, prec); -1 for a bad kind.  No counterpart in the
+ * reference. */
+int BQCget_tile(int kind, int cls, int prec, int nao);
 
 /* Task list of one build on the host (the per-task loop of
  * cuobc/scf/hf.py:111-145 with _make_s_index_offsets, hf.py:567-578).

@@ -103,9 +103,27 @@ def test_autotuned_tiles_keep_results():
         base = bj.build(dms)
         table = autotune.tune(lambda: bj.build(dms), candidates={1: (16, 256), 4: (8, 128)})
         assert table and all(rec['ck'] in rec['ms'] for rec in table.values())
+        for (kind, cls, prec), rec in table.items():
+            # the tuned tile is installed; reset() brings back the default one
+            assert autotune.current_tile(kind, cls, prec, 100) == rec['ck']
+            assert rec['builtin'] in (16, 32, 64, 256)
         got = bj.build(dms)
         for a, b in zip(base, got):
             assert _maxdiff(a[0].cpu(), b[0].cpu()) < 1e-11
             assert _maxdiff(a[1].cpu(), b[1].cpu()) < 1e-11
+    finally:
+        autotune.reset()
+    from byteqc_amd.lib import load_library
+    lib = load_library()
+    # default tiles by molecule size: s/p rows, 108-component rows, waves
+    assert [lib.BQCget_tile(1, 1000, 0, n) for n in (500, 2624)] == [32, 256]
+    assert [lib.BQCget_tile(1, 2120, 1, n) for n in (500, 2624)] == [16, 64]
+    assert [lib.BQCget_tile(4, 2111, 0, n) for n in (500, 10368)] == [16, 64]
+    assert lib.BQCget_tile(2, 0, 0, 100) == -1
+    lib.BQCset_tile(1, -1, 0, 128)          # a tile for all classes of a kind beats the defaults
+    lib.BQCset_tile(1, 1000, 0, 16)         # and a per-class tile beats that
+    try:
+        assert all(lib.BQCget_tile(1, c, p, 3000) == 128 for c in (0, 2120) for p in (0, 1))
+        assert lib.BQCget_tile(1, 1000, 0, 3000) == 16
     finally:
         autotune.reset()

@@ -222,3 +222,4 @@ def test_fractional_aufbau_occupations():
     occ = _frac_occ(e, 10)
     assert numpy.allclose(occ, [0, 2, 2, 2, 2, 2, 0]) and occ.sum() == 10
     assert numpy.allclose(_frac_occ(e, 11).sum(), 11)
+
