@@ -380,6 +380,7 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
     if (!bra_order) return -1;
     RowKernel k[2];
     int wv[2];
+    bool fp32 = true;
     for (int prec = 0; prec < 2; ++prec) {
         RowKernel r = c.swap ? g_rows[prec][c.pkl][c.pij] : g_rows[prec][c.pij][c.pkl];
         RowKernel w = c.swap ? g_waves[prec][c.pkl][c.pij] : g_waves[prec][c.pij][c.pkl];
@@ -389,10 +390,16 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
         } else if (w && scratch_ok((const void *)w)) {
             k[prec] = w;
             wv[prec] = 1;
+        } else if (prec == 1 && !wv[0]) {
+            k[1] = k[0];
+            wv[1] = 0;
+            fp32 = false;
         } else {
             return -1;
         }
     }
+    // a row class without an FP32 instance runs all its quartets in FP64
+    const double log_cut = fp32 ? c.log_cut : -1e300;
     BasisProdCache *bp = c.bp;
     const int nbins = c.nbins;
     const int *pbins = c.swap ? c.bins_kl : c.bins_ij;
@@ -409,7 +416,7 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
         for (int bpn = 0; bpn < nbins - bq; ++bpn) {
             int p0 = pbins[bpn], p1 = pbins[bpn + 1];
             if (p1 <= p0) continue;
-            int prec = bin_ub(c, cp_p, pbins, bpn) + ubq < c.log_cut ? 1 : 0;
+            int prec = bin_ub(c, cp_p, pbins, bpn) + ubq < log_cut ? 1 : 0;
             nquart[prec] += count_rect(p0, p1, q0, q1, c.diag);
         }
     }
@@ -463,7 +470,7 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
         double ubp = bin_ub(c, cp_p, pbins, s);
         int split = nbins - s;
         for (int bq = 0; bq < nbins - s; ++bq)
-            if (ubp + bin_ub(c, cp_q, qbins, bq) < c.log_cut) {
+            if (ubp + bin_ub(c, cp_q, qbins, bq) < log_cut) {
                 split = bq;
                 break;
             }

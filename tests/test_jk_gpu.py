@@ -187,9 +187,10 @@ def test_atomic_densities_hold_the_atoms_electrons():
 
 
 def test_fp32_everywhere_cc_pvtz_stays_at_fp32_accuracy():
-    """All quartets in FP32 (cut 1e30), every class to (ff|ff): J and K stay
-    within FP32 rounding of the FP64 result (catches a broken FP32 kernel
-    that the 1e-7 split would hide behind small integrals)."""
+    """All quartets of every class with an FP32 instance in FP32 (cut 1e30),
+    classes to (ff|ff): J and K stay within FP32 rounding of the FP64 result
+    (catches a broken FP32 kernel that the 1e-7 split would hide behind small
+    integrals)."""
     _torch_gpu()
     from byteqc_amd.gto import M
     from byteqc_amd.scf import hf
