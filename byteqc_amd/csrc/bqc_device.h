@@ -94,6 +94,37 @@ __device__ __forceinline__ void rys_roots(double x, double *__restrict__ u,
     rys_roots<N>(x, u, w, (const double *)(BQC_RYS_COEF + rys_offset(N)));
 }
 
+// Root r (0 <= r < N) and its weight only: the same table and asymptotic
+// form as rys_roots, two Chebyshev sums instead of 2N (lanes that need
+// different roots of one x run the same instructions).
+template <int N>
+__device__ __forceinline__ void rys_root_one(double x, int r, double &u, double &w) {
+    static_assert(N >= 1 && N <= BQC_RYS_NMAX, "unsupported number of roots");
+    if (x >= (double)rys_xbig(N)) {
+        const double h = BQC_RYS_HERM_S[(N - 1) * BQC_RYS_NMAX + r];
+        u = h / (x - h);
+        w = BQC_RYS_HERM_W[(N - 1) * BQC_RYS_NMAX + r] * rsqrt(x);
+        return;
+    }
+    const int it = (int)x;
+    const double y = 2.0 * (x - (double)it) - 1.0;
+    const double y2 = 2.0 * y;
+    const double *c = (const double *)(BQC_RYS_COEF + rys_offset(N)) + it * (2 * N * (BQC_RYS_DEG + 1));
+    const double *cu = c + r * (BQC_RYS_DEG + 1), *cw = c + (N + r) * (BQC_RYS_DEG + 1);
+    double bu1 = 0.0, bu2 = 0.0, bw1 = 0.0, bw2 = 0.0;
+#pragma unroll
+    for (int j = BQC_RYS_DEG; j >= 1; --j) {
+        const double tu = fma(y2, bu1, cu[j] - bu2);
+        const double tw = fma(y2, bw1, cw[j] - bw2);
+        bu2 = bu1;
+        bu1 = tu;
+        bw2 = bw1;
+        bw1 = tw;
+    }
+    u = fma(y, bu1, cu[0] - bu2);
+    w = fma(y, bw1, cw[0] - bw2);
+}
+
 // Class traits of a quartet (LI LJ | LK LL).
 template <int LI, int LJ, int LK, int LL>
 struct Cls {

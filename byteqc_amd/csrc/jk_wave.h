@@ -241,15 +241,8 @@ __global__ void __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>()))
                         // phase A: one (direction, root) table per lane
                         if (lane < 3 * NR) {
                             const int d = lane % 3, r = lane / 3;
-                            double u[NR], w[NR];
-                            rys_roots<NR>(x, u, w);
-                            double ur = u[0], wr = w[0];
-#pragma unroll
-                            for (int k = 1; k < NR; ++k)
-                                if (r == k) {
-                                    ur = u[k];
-                                    wr = w[k];
-                                }
+                            double ur, wr;
+                            rys_root_one<NR>(x, r, ur, wr);
                             const double u2 = a0q * ur;
                             const double tmp4 = 0.5 / (u2 * aijkl + a1);
                             const double b00 = u2 * tmp4;
