@@ -160,7 +160,8 @@ void BQCgraph_destroy(void *graph);
  * on the molecule's Cartesian AO count (measured on MI355X: rows 256 / 64
  * for classes above 81 components / waves 64 from 2048 AOs up, else
  * 32 / 16 / 16).  Precedence when a build is planned: per-class setting,
- * all-class setting, default.  Process-wide; used by the tile autotuner.
+ * all-class setting, default.  Process-wide; used by the tile autotuner;
+ * not to be called while another thread plans a build.
  * No counterpart in the reference (its launch shapes are fixed per class,
  * nr_jk_driver.cu:46-225).  Returns 0 ok, 1 bad argument. */
 int BQCset_tile(int kind, int cls, int prec, int ck);
