@@ -95,11 +95,11 @@ static double quartet_flops(int li, int lj, int lk, int ll, int K) {
 static double count_rect(int i0, int i1, int k0, int k1, bool diag) {
     if (i1 <= i0 || k1 <= k0) return 0;
     if (!diag) return (double)(i1 - i0) * (k1 - k0);
-    double n = 0;
-    for (int kl = k0; kl < k1; ++kl) {
-        int lo = i0 > kl ? i0 : kl;
-        if (i1 > lo) n += i1 - lo;
-    }
+    // kl < i0 meets all i1 - i0 pairs; i0 <= kl < i1 meets i1 - kl; kl >= i1 none
+    const long long below = (long long)(k1 < i0 ? k1 : i0) - k0;
+    double n = below > 0 ? (double)below * (i1 - i0) : 0.0;
+    const long long a = k0 > i0 ? k0 : i0, b = k1 < i1 ? k1 : i1;
+    if (b > a) n += (double)(b - a) * i1 - (double)(a + b - 1) * (b - a) / 2;
     return n;
 }
 
