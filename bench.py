@@ -354,7 +354,8 @@ def run_serve(args, torch):
     srv.submit(water_mole(3, box_for_density(3), 1, 'def2-svp'))
     srv.drain()
     srv.close()
-    srv = Server(log=log)
+    lanes = tuple(int(x) for x in args.lanes.split(',')) if args.lanes else (800, 1600)
+    srv = Server(log=log, lanes=lanes)
     reqs, wall = run_load(srv, arr, lambda n, sd: water_mole(n, box_for_density(n), sd,
                                                             'def2-svp'), log=log)
     srv.close()
@@ -372,7 +373,7 @@ def run_serve(args, torch):
                                f'{args.duration} s, {args.min_water}-{args.max_water} H2O '
                                'log-uniform, def2-svp, mixed J/K until |dE|<1e-5 then FP64, '
                                'converged to 1e-8 Eh',
-                   'requests': len(reqs), 'parallelism': 'single GPU'},
+                   'requests': len(reqs), 'lanes': list(lanes), 'parallelism': 'single GPU'},
         'serving': {'offered_qps': args.qps, 'completed_qps': len(reqs) / wall,
                     'wall_s': wall, 'converged': len(ok), 'requests': len(reqs),
                     'latency_p50_s': _percentile(lat, 50), 'latency_p99_s': _percentile(lat, 99),
@@ -396,6 +397,7 @@ def main():
     ap.add_argument('--workload', default='c2', choices=('c1', 'c2', 'c3', 'c4', 'c5'))
     ap.add_argument('--qps', type=float, default=0.5, help='c5: request rate')
     ap.add_argument('--duration', type=float, default=60.0, help='c5: seconds of arrivals')
+    ap.add_argument('--lanes', default='', help='c5: AO counts separating the size lanes')
     ap.add_argument('--min-water', type=int, default=5)
     ap.add_argument('--max-water', type=int, default=120)
     ap.add_argument('--precision', default='fp64', choices=('fp64', 'mixed'))
