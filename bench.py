@@ -16,8 +16,11 @@ from a captured HIP graph; a step is one build of all 256 molecules and
 ``value`` is molecule builds/s.  ``--precision mixed`` computes quartets
 whose Schwarz x density bound is below 1e-7 in FP32 (FP64 roots, prefactors
 and accumulators) and reports |dE| (E = 1/2 Tr D(J - K/2)), max|dJ|, max|dK|
-against an FP64 build of the same input.  ``--autotune`` tunes the per-class
-tiles before timing.
+against an FP64 build of the same input.  Work-item tiles default by
+molecule size; ``--autotune`` tunes them per class before timing.  ``c5``
+feeds Poisson arrivals of RHF requests (5-120 waters, def2-SVP) into the
+serving loop (``--qps``, ``--duration``, ``--lanes``) and reports latency
+percentiles.
 
 Prints one JSON line.  ``roofline`` is for the kernel class that takes the
 most time: its algorithmic FLOPs per launch (quartet count x the per-quartet
