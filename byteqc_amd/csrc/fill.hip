@@ -51,31 +51,6 @@ extern "C" int GINTfill_int2e(BasisProdCache *bp, double *eri, int nao, const in
         attr.localSizeBytes > BQC_FILL_MAX_SCRATCH)
         return 1;
 
-    TaskArgs ta;
-    memset(&ta, 0, sizeof(ta));
-    const int cp_p = swap ? cp_kl_id : cp_ij_id, cp_q = swap ? cp_ij_id : cp_kl_id;
-    ta.bas_p0 = bp->bas_pairs_locs[cp_p];
-    ta.bas_q0 = bp->bas_pairs_locs[cp_q];
-    ta.prim_p0 = bp->primitive_pairs_locs[cp_p];
-    ta.prim_q0 = bp->primitive_pairs_locs[cp_q];
-    ta.nprim_p = bp->cptype[cp_p].nprim_12;
-    ta.nprim_q = bp->cptype[cp_q].nprim_12;
-    ta.swap = swap;
-    ta.diag = 0;
-    ta.pre[0] = 0;
-    // kl bin b meets the ij pairs of bins [0, nbins - b)
-    for (int b = 0; b < nbins; ++b) {
-        const int k0 = bins_locs_kl[b], k1 = bins_locs_kl[b + 1];
-        const int i0 = bins_locs_ij[0], i1 = bins_locs_ij[nbins - b];
-        if (k1 <= k0 || i1 <= i0) continue;
-        const int r = ta.nbins;
-        ta.klb[r] = k0;
-        ta.ij0[r] = i0;
-        ta.nij[r] = i1 - i0;
-        ta.pre[r + 1] = ta.pre[r] + (long long)(k1 - k0) * (i1 - i0);
-        ta.nbins = r + 1;
-    }
-    if (ta.nbins == 0) return 0;
     EriOut out;
     out.data = eri;
     for (int d = 0; d < 4; ++d) {
@@ -94,8 +69,45 @@ extern "C" int GINTfill_int2e(BasisProdCache *bp, double *eri, int nao, const in
     pd.bra = bp->bas_pair2bra;
     pd.ket = bp->bas_pair2ket;
     pd.ao_loc = bp->ao_loc;
-    const long long nblk = (ta.pre[ta.nbins] + BQC_THREADS - 1) / BQC_THREADS;
-    if (nblk > 0x7fffffffLL) return 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, out, pd, ta);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    TaskArgs ta;
+    const int cp_p = swap ? cp_kl_id : cp_ij_id, cp_q = swap ? cp_ij_id : cp_kl_id;
+    auto reset = [&]() {
+        memset(&ta, 0, sizeof(ta));
+        ta.bas_p0 = bp->bas_pairs_locs[cp_p];
+        ta.bas_q0 = bp->bas_pairs_locs[cp_q];
+        ta.prim_p0 = bp->primitive_pairs_locs[cp_p];
+        ta.prim_q0 = bp->primitive_pairs_locs[cp_q];
+        ta.nprim_p = bp->cptype[cp_p].nprim_12;
+        ta.nprim_q = bp->cptype[cp_q].nprim_12;
+        ta.swap = swap;
+        ta.diag = 0;
+        ta.pre[0] = 0;
+    };
+    auto launch = [&]() -> int {
+        if (ta.nbins == 0) return 0;
+        const long long nblk = (ta.pre[ta.nbins] + BQC_THREADS - 1) / BQC_THREADS;
+        if (nblk > 0x7fffffffLL) return 1;
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, out, pd,
+                           ta);
+        return hipGetLastError() == hipSuccess ? 0 : 1;
+    };
+    reset();
+    // kl bin b meets the ij pairs of bins [0, nbins - b); at most
+    // BQC_ARG_BINS rectangles per launch
+    for (int b = 0; b < nbins; ++b) {
+        const int k0 = bins_locs_kl[b], k1 = bins_locs_kl[b + 1];
+        const int i0 = bins_locs_ij[0], i1 = bins_locs_ij[nbins - b];
+        if (k1 <= k0 || i1 <= i0) continue;
+        const int r = ta.nbins;
+        ta.klb[r] = k0;
+        ta.ij0[r] = i0;
+        ta.nij[r] = i1 - i0;
+        ta.pre[r + 1] = ta.pre[r] + (long long)(k1 - k0) * (i1 - i0);
+        ta.nbins = r + 1;
+        if (ta.nbins == BQC_ARG_BINS) {
+            if (launch()) return 1;
+            reset();
+        }
+    }
+    return launch();
 }

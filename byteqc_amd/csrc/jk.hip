@@ -203,21 +203,22 @@ static bool scratch_ok(const void *kern) {
 }
 
 // Generic kernels (one lane per quartet): rectangle kl-bin b x ij in
-// [0, bins_ij[nbins-b]) split at the first FP32 ij bin.
+// [0, bins_ij[nbins-b]) split at the first FP32 ij bin; at most
+// BQC_ARG_BINS rectangles per launch.
 static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &pd,
                            hipStream_t stream, double *flops) {
     BasisProdCache *bp = c.bp;
     const int nbins = c.nbins;
     const int *bins_ij = c.bins_ij, *bins_kl = c.bins_kl;
+    const int *locs = bp->bas_pairs_locs, *plocs = bp->primitive_pairs_locs;
+    const int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
     TaskArgs ta[2];
     double nq[2] = {0, 0};
-    for (int prec = 0; prec < 2; ++prec) {
+    auto reset = [&](int prec) {
         TaskArgs &t = ta[prec];
         memset(&t, 0, sizeof(t));
         t.swap = c.swap;
         t.diag = c.diag;
-        const int *locs = bp->bas_pairs_locs, *plocs = bp->primitive_pairs_locs;
-        int cp_p = c.swap ? c.cp_kl : c.cp_ij, cp_q = c.swap ? c.cp_ij : c.cp_kl;
         t.bas_p0 = locs[cp_p];
         t.bas_q0 = locs[cp_q];
         t.prim_p0 = plocs[cp_p];
@@ -225,7 +226,36 @@ static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &p
         t.nprim_p = bp->cptype[cp_p].nprim_12;
         t.nprim_q = bp->cptype[cp_q].nprim_12;
         t.pre[0] = 0;
-    }
+        nq[prec] = 0;
+    };
+    auto launch = [&](int prec) -> int {
+        TaskArgs &t = ta[prec];
+        if (t.nbins == 0) return 0;
+        const double lflops = nq[prec] * c.qflops;
+        flops[prec] += lflops;
+        JKKernel kern = c.swap ? g_kernels[prec][c.pkl][c.pij] : g_kernels[prec][c.pij][c.pkl];
+        if (!kern) return 1;
+        if (!scratch_ok((const void *)kern)) {
+            fprintf(stderr, "bqcjk: (%d%d|%d%d) kernel needs more scratch than allowed\n", c.li,
+                    c.lj, c.lk, c.ll);
+            return 1;
+        }
+        long long nblk = (t.pre[t.nbins] + BQC_THREADS - 1) / BQC_THREADS;
+        if (nblk > 0x7fffffffLL) return 1;
+        LaunchRec rec;
+        if (g_prof) record_launch(rec, eval_cls(c), 2 * prec, lflops, nq[prec], stream);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, t);
+        if (g_prof) keep_launch(rec, stream);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) {
+            fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", c.li, c.lj, c.lk, c.ll,
+                    hipGetErrorString(err));
+            return 1;
+        }
+        return 0;
+    };
+    reset(0);
+    reset(1);
     for (int b = 0; b < nbins; ++b) {
         int k0 = bins_kl[b], k1 = bins_kl[b + 1];
         int iend_bin = nbins - b;
@@ -249,32 +279,14 @@ static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &p
             t.pre[r + 1] = t.pre[r] + (long long)(k1 - k0) * (i1 - i0);
             t.nbins = r + 1;
             nq[prec] += count_rect(i0, i1, k0, k1, c.diag);
+            if (t.nbins == BQC_ARG_BINS) {
+                if (launch(prec)) return 1;
+                reset(prec);
+            }
         }
     }
-    for (int prec = 0; prec < 2; ++prec) {
-        TaskArgs &t = ta[prec];
-        if (t.nbins == 0) continue;
-        const double lflops = nq[prec] * c.qflops;
-        flops[prec] += lflops;
-        JKKernel kern = c.swap ? g_kernels[prec][c.pkl][c.pij] : g_kernels[prec][c.pij][c.pkl];
-        if (!kern) return 1;
-        if (!scratch_ok((const void *)kern)) {
-            fprintf(stderr, "bqcjk: (%d%d|%d%d) kernel needs more scratch than allowed\n", c.li,
-                    c.lj, c.lk, c.ll);
-            return 1;
-        }
-        long long nblk = (t.pre[t.nbins] + BQC_THREADS - 1) / BQC_THREADS;
-        LaunchRec rec;
-        if (g_prof) record_launch(rec, eval_cls(c), 2 * prec, lflops, nq[prec], stream);
-        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, t);
-        if (g_prof) keep_launch(rec, stream);
-        hipError_t err = hipGetLastError();
-        if (err != hipSuccess) {
-            fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", c.li, c.lj, c.lk, c.ll,
-                    hipGetErrorString(err));
-            return 1;
-        }
-    }
+    for (int prec = 0; prec < 2; ++prec)
+        if (launch(prec)) return 1;
     return 0;
 }
 
@@ -341,7 +353,7 @@ struct Group {
 
 struct BuildPlan {
     std::vector<TaskDesc> tasks;
-    std::vector<size_t> perm_p_off, perm_q_off;  // in ints
+    std::vector<size_t> perm_p_off, perm_q_off, bins_off;  // in ints
     std::vector<int> ints;                       // list orders
     std::vector<Group> groups;
     double flops[2] = {0, 0};
@@ -441,11 +453,10 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
     td.nprim_q = bp->cptype[cp_q].nprim_12;
     td.diag = c.diag;
     td.nbins = nbins;
-    for (int b = 0; b <= nbins; ++b) {
-        td.pbins[b] = pbins[b];
-        td.qbins[b] = qbins[b];
-    }
     bp_.tasks.push_back(td);
+    bp_.bins_off.push_back(bp_.ints.size());
+    bp_.ints.insert(bp_.ints.end(), pbins, pbins + nbins + 1);
+    bp_.ints.insert(bp_.ints.end(), qbins, qbins + nbins + 1);
     size_t off_p = bp_.ints.size();
     side_order(pbins, nbins, bra_order + locs[cp_p], locs[cp_p + 1] - locs[cp_p], bp_.ints);
     size_t off_q = off_p;
@@ -533,6 +544,8 @@ static void stage_fill(BuildPlan &p, const Staged &st, char *dev, char *host) {
     for (size_t t = 0; t < p.tasks.size(); ++t) {
         p.tasks[t].perm_p = perm_dev + p.perm_p_off[t];
         p.tasks[t].perm_q = perm_dev + p.perm_q_off[t];
+        p.tasks[t].pbins = perm_dev + p.bins_off[t];
+        p.tasks[t].qbins = perm_dev + p.bins_off[t] + p.tasks[t].nbins + 1;
     }
     memcpy(host, p.tasks.data(), p.tasks.size() * sizeof(TaskDesc));
     memcpy(host + st.off_perm, p.ints.data(), p.ints.size() * sizeof(int));
@@ -563,18 +576,27 @@ static int launch_groups(const BuildPlan &p, const Staged &st, char *dev, hipStr
     return 0;
 }
 
-// Device image of the work list of direct (non-graph) builds, one per
-// stream.  Builds on one stream run in order, so a build may refill the
-// image once its copy is queued behind the previous build's kernels; builds
-// on different streams have separate images and may overlap on the device.
+// Device images of the work lists of direct (non-graph) builds: a ring of
+// BQC_STAGE_SLOTS (pinned host buffer, device buffer) pairs per stream.
+// Builds on one stream run in order, so a slot's device buffer may be
+// refilled by a copy queued behind the kernels of earlier builds; its host
+// buffer only once the previous copy out of it has completed, which lets the
+// host plan a build while the device still runs the one before (the
+// reference-ABI entry GINTbuild_jk is called once per task).  Builds on
+// different streams have separate rings and may overlap on the device.
 // One host thread drives a given stream at a time.
-struct StreamStage {
+#define BQC_STAGE_SLOTS 2
+struct StageSlot {
     char *dws = nullptr;
     size_t dws_cap = 0;
     char *hstage = nullptr;
     size_t hstage_cap = 0;
     hipEvent_t evt = nullptr;
     bool pending = false;
+};
+struct StreamStage {
+    StageSlot slot[BQC_STAGE_SLOTS];
+    int next = 0;
 };
 static std::mutex g_stage_mu;
 static std::map<hipStream_t, StreamStage> g_stages;
@@ -587,9 +609,11 @@ static StreamStage &stage_for(hipStream_t stream) {
 static int stage_and_launch(BuildPlan &p, hipStream_t stream) {
     if (p.groups.empty()) return 0;
     Staged st = stage_layout(p);
-    StreamStage &ss = stage_for(stream);
+    StreamStage &sst = stage_for(stream);
+    StageSlot &ss = sst.slot[sst.next];
+    sst.next = (sst.next + 1) % BQC_STAGE_SLOTS;
     if (ss.pending) {
-        // the previous copy from the pinned buffer must be done before refilling it
+        // the previous copy from this slot's pinned buffer must be done
         hipEventSynchronize(ss.evt);
         ss.pending = false;
     }
@@ -645,6 +669,8 @@ static int plan_molecules(BuildPlan &plan, const MolArgs *m, int nmol, int nbins
                           std::vector<Deferred> *deferred = nullptr) {
     if (precision != 0 && precision != 1) return 1;
     if (nbins > BQC_MAX_BINS) return 3;
+    for (int im = 0; im < nmol; ++im)
+        if (m[im].nao > BQC_MAX_NAO) return 4;
     register_kernels();
     for (int im = 0; im < nmol; ++im) {
         const MolArgs &a = m[im];
@@ -669,20 +695,84 @@ static int plan_molecules(BuildPlan &plan, const MolArgs *m, int nmol, int nbins
     return 0;
 }
 
+// First-shell order of the pair lists of a plan, for the reference-ABI
+// entry GINTbuild_jk (whose caller has no such array): per pair type the
+// plan positions sorted by first shell, stable (a counting sort of the
+// host bra row of bas_pair2shls, which the cache aliases).  Built per pair
+// type on first use and kept per cache until BQCdel_basis_prod_clone or
+// GINTdel_basis_prod forgets it.
+struct PairOrder {
+    const int *pair2shls = nullptr;
+    int npairs = 0, ncptype = 0;
+    std::vector<int> order;
+    std::vector<char> done;
+};
+static std::mutex g_order_mu;
+static std::map<const BasisProdCache *, PairOrder> g_orders;
+
+static const int *first_shell_order(const BasisProdCache *bp, int cp_a, int cp_b) {
+    const int *locs = bp->bas_pairs_locs;
+    const int npairs = locs[bp->ncptype];
+    std::lock_guard<std::mutex> lock(g_order_mu);
+    PairOrder &po = g_orders[bp];
+    if (po.pair2shls != bp->bas_pair2shls || po.npairs != npairs ||
+        po.ncptype != bp->ncptype) {
+        po.pair2shls = bp->bas_pair2shls;
+        po.npairs = npairs;
+        po.ncptype = bp->ncptype;
+        po.order.assign(npairs > 0 ? npairs : 1, 0);
+        po.done.assign(bp->ncptype, 0);
+    }
+    const int cps[2] = {cp_a, cp_b};
+    for (int cp : cps) {
+        if (po.done[cp]) continue;
+        const int p0 = locs[cp], n = locs[cp + 1] - locs[cp];
+        const int *bra = bp->bas_pair2shls + p0;
+        int lo = 0, hi = -1;
+        for (int k = 0; k < n; ++k) {
+            if (k == 0 || bra[k] < lo) lo = bra[k];
+            if (bra[k] > hi) hi = bra[k];
+        }
+        std::vector<int> cnt(n > 0 ? hi - lo + 2 : 1, 0);
+        for (int k = 0; k < n; ++k) ++cnt[bra[k] - lo + 1];
+        for (size_t v = 1; v < cnt.size(); ++v) cnt[v] += cnt[v - 1];
+        for (int k = 0; k < n; ++k) po.order[p0 + cnt[bra[k] - lo]++] = k;
+        po.done[cp] = 1;
+    }
+    return po.order.data();
+}
+
+extern "C" void bqc_forget_pair_order(const BasisProdCache *bp) {
+    std::lock_guard<std::mutex> lock(g_order_mu);
+    g_orders.erase(bp);
+}
+
 extern "C" {
 
 int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                  int n_dm, int *bins_locs_ij, int *bins_locs_kl, int nbins, int cp_ij_id,
                  int cp_kl_id, hipStream_t stream) {
     register_kernels();
+    if (nao > BQC_MAX_NAO) return 4;
+    if (nbins < 1 || cp_ij_id < 0 || cp_kl_id < 0 || cp_ij_id >= bpcache->ncptype ||
+        cp_kl_id >= bpcache->ncptype)
+        return 1;
     JKArgs jk = {vj, vk, dm, nao, n_dm};
     PairData pd = make_pair_data(bpcache);
     TaskCtx c;
     int err = task_init(c, bpcache, bins_locs_ij, bins_locs_kl, nbins, cp_ij_id, cp_kl_id,
                         nullptr, 0, 0, 0);
     if (err) return err;
-    double fl[2] = {0, 0};
-    return launch_quartets(c, jk, pd, stream, fl);
+    const int *order = bpcache->bas_pair2shls
+                           ? first_shell_order(bpcache, cp_ij_id, cp_kl_id)
+                           : nullptr;
+    BuildPlan plan;
+    err = plan_task(plan, c, order, jk, pd);
+    if (err < 0) err = launch_quartets(c, jk, pd, stream, plan.flops);
+    else if (err == 0) err = stage_and_launch(plan, stream);
+    g_last_flops[0] = plan.flops[0];
+    g_last_flops[1] = plan.flops[1];
+    return err;
 }
 
 int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,

@@ -30,8 +30,15 @@
 
 using namespace bqc;
 
-#define BQC_MAX_BINS 32
+// Schwarz bins per pair list of one task (a sanity limit: the bin offsets of
+// the row and wave kernels live in the device work list)
+#define BQC_MAX_BINS 4096
+// Rectangles per launch of the generic kernels (their arguments travel by
+// value; a task with more non-empty rectangles takes several launches)
+#define BQC_ARG_BINS 32
 #define BQC_THREADS 256
+// Largest Cartesian AO count: J/K/D element offsets are 32-bit in the kernels
+#define BQC_MAX_NAO 46340
 
 struct TaskArgs {
     int bas_p0, bas_q0;    // first global pair index of the p / q side
@@ -40,10 +47,10 @@ struct TaskArgs {
     int diag;              // cp_ij == cp_kl
     int swap;              // p side is the task's kl list
     int nbins;
-    int klb[BQC_MAX_BINS + 1];
-    int ij0[BQC_MAX_BINS];
-    int nij[BQC_MAX_BINS];
-    long long pre[BQC_MAX_BINS + 1];
+    int klb[BQC_ARG_BINS + 1];
+    int ij0[BQC_ARG_BINS];
+    int nij[BQC_ARG_BINS];
+    long long pre[BQC_ARG_BINS + 1];
 };
 
 struct JKArgs {
@@ -73,8 +80,7 @@ struct TaskDesc {
     int np;                      // P pairs in bins
     int bas_p0, bas_q0, prim_p0, prim_q0, nprim_p, nprim_q;
     int diag, nbins;
-    int pbins[BQC_MAX_BINS + 1];
-    int qbins[BQC_MAX_BINS + 1];
+    const int *pbins, *qbins;    // nbins + 1 bin offsets of each side
 };
 
 __device__ __forceinline__ void atomic_add(double *p, double v) {

@@ -114,9 +114,13 @@ void GINTinit_basis_prod_gpu(BasisProdCache **in, BasisProdCache **out, int *ao_
     *out = dst;
 }
 
+// Drops the first-shell order the J/K driver keeps per cache (jk.hip).
+void bqc_forget_pair_order(const BasisProdCache *bp);
+
 void GINTdel_basis_prod(BasisProdCache **pbp) {
     BasisProdCache *bp = *pbp;
     if (!bp) return;
+    bqc_forget_pair_order(bp);
     free(bp->cptype);
     free(bp->primitive_pairs_locs);
     free(bp->aexyz);
@@ -125,6 +129,7 @@ void GINTdel_basis_prod(BasisProdCache **pbp) {
 }
 
 void BQCdel_basis_prod_clone(BasisProdCache **pbp) {
+    if (*pbp) bqc_forget_pair_order(*pbp);
     if (*pbp) free(*pbp);
     *pbp = NULL;
 }

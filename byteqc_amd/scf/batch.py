@@ -83,7 +83,7 @@ class BatchJK:
         tcp = arr([p[0].ctypes.data for p in per])
         tbins = arr([p[1].ctypes.data for p in per])
         bra = arr([o.bra_order.ctypes.data for o in self.opts])
-        lq = arr([o.log_q.ctypes.data for o in self.opts])
+        lq = arr([o.log_q_prec.ctypes.data for o in self.opts])
         tdm = arr([p[2].ctypes.data for p in per])
         keep.append(per)
         return keep, (n, bps, vj, vk, dm, nao, 1, ntask, tcp, tbins, self.nbins, bra,

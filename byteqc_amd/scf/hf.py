@@ -228,6 +228,18 @@ def _split_l_ctr_groups(uniq_l_ctr, l_ctr_counts, group_size):
 # ---------------------------------------------------------------------------
 # plan (cuobc/scf/hf.py:277-441)
 # ---------------------------------------------------------------------------
+# J/K, density and their element offsets are 32-bit inside the kernels
+MAX_CART_AO = 46340
+
+
+def check_nao(nao_c):
+    """Refuse a basis whose decontracted Cartesian AO count exceeds the
+    kernels' 32-bit element offsets (nao_c**2 must stay below 2**31)."""
+    if nao_c > MAX_CART_AO:
+        raise ValueError(f'{nao_c} decontracted Cartesian AOs: this J/K engine supports at '
+                         f'most {MAX_CART_AO} (32-bit element offsets)')
+
+
 class _VHFOpt:
     def __init__(self, mol, intor='int2e', prescreen='CVHFnoscreen',
                  qcondname='CVHFsetnr_direct_scf', dmcondname=None):
@@ -239,6 +251,7 @@ class _VHFOpt:
 
     def build(self, cutoff=1e-13, group_size=None, diag_block_with_triu=False, gpus=None):
         t0 = time.perf_counter()
+        check_nao(int(self.mol.ao_loc_nr(cart=True)[-1]))
         dev = _device()
         lib = load_library()
         mol = self.mol
@@ -281,7 +294,7 @@ class _VHFOpt:
         q_cond = q_d.cpu().numpy()
         self.q_cond = q_cond
 
-        log_qs, bra, ket, cp_l, cp_grp = [], [], [], [], []
+        log_qs, log_raw, bra, ket, cp_l, cp_grp = [], [], [], [], [], []
         offs = self.l_ctr_offsets
         for i in range(len(uniq_l_ctr)):
             p0, p1 = offs[i], offs[i + 1]
@@ -297,11 +310,15 @@ class _VHFOpt:
                 bra.append(ish[mask] + p0)
                 ket.append(jsh[mask] + q0)
                 lq = numpy.log(qs[mask])
+                log_raw.append(lq.copy())
                 lq[lq > 0] = 0
                 log_qs.append(lq)
                 cp_l.append((int(uniq_l_ctr[i, 0]), int(uniq_l_ctr[j, 0])))
                 cp_grp.append((i, j))
         self.log_qs = log_qs
+        # unclipped log q of every pair (plan order): the bound of the
+        # mixed-precision split, where a pair with q > 1 must count fully
+        self.log_q_prec = numpy.ascontiguousarray(numpy.concatenate(log_raw + [numpy.zeros(1)]))
         self.cp_l = cp_l
         self.cp_grp = numpy.ascontiguousarray(numpy.asarray(cp_grp, dtype=numpy.int32).reshape(-1, 2))
         self.bas_pair2shls = numpy.ascontiguousarray(
@@ -468,7 +485,7 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None,
     if len(task_cp):
         err = lib.BQCbuild_jk(vhfopt.bpcache, _ptr(vj), _ptr(vk), _ptr(dmc), nao, n_dm,
                               len(task_cp), _ptr(task_cp), _ptr(task_bins), nbins,
-                              _ptr(vhfopt.bra_order), PRECISION[precision], _ptr(vhfopt.log_q),
+                              _ptr(vhfopt.bra_order), PRECISION[precision], _ptr(vhfopt.log_q_prec),
                               _ptr(task_dmax), fp32_cut, s)
         if err:
             bad = []

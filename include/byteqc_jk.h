@@ -86,7 +86,14 @@ void GINTdel_basis_prod(BasisProdCache **pbp);
  * bins_locs_ij / bins_locs_kl (host, nbins+1) split each pair list into
  * Schwarz bins; kl pairs of bin b meet ij pairs of bins [0, nbins-b).
  * vj / vk / dm: device, n_dm x nao x nao (vj or vk may be NULL).
- * Returns 0 ok, 1 launch error or unsupported class, 2 nroots >= 12. */
+ * The quartets run in the same row and wavefront kernels as BQCbuild_jk;
+ * the pair lists are walked grouped by first shell, an order built from the
+ * host bra row of bas_pair2shls (aliased by the cache) on first use of each
+ * pair type and kept until the cache is deleted.  Calls on one stream are
+ * ordered; the host may plan the next task while the device runs this one.
+ * Returns 0 ok, 1 launch error or unsupported class, 2 nroots >= 12; codes
+ * this library adds: 3 more than 4096 bins, 4 nao above 46340 (element
+ * offsets are 32-bit). */
 int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                  int n_dm, int *bins_locs_ij, int *bins_locs_kl, int nbins, int cp_ij_id,
                  int cp_kl_id, hipStream_t stream);
@@ -114,15 +121,20 @@ void BQCdel_basis_prod_clone(BasisProdCache **pbp);
  * then kl bins per task, as for GINTbuild_jk.  bra_order (host, one entry
  * per pair, or NULL): for every pair type, the positions of its pairs in
  * the pair list ordered by first shell (stable); it lets the kernels walk
- * each Schwarz bin grouped by first shell.  Builds that share a library
- * must be ordered on one stream (they share a device work list).  precision: 0 = FP64
+ * each Schwarz bin grouped by first shell.  The library keeps its device
+ * work lists per HIP stream: builds on one stream are ordered, builds on
+ * different streams may overlap on the device, one host thread drives a
+ * stream at a time, and BQCset_tile must not run concurrently with a build.
+ * precision: 0 = FP64
  * throughout; 1 = mixed: a quartet is evaluated in FP32 (Rys roots and
  * weights, prefactors and all J/K accumulation stay FP64) when the upper
  * bound q_P q_Q dmax of its Schwarz bins, taken over the 64 pairs sharing a
  * wavefront, falls below fp32_cut.  log_q (host, one per pair, plan order)
  * gives log q; task_dmax (host, one per task) the task's density bound.
+ * log_q is used for this split only (the bins come from task_bins), so it
+ * may hold the unclipped log q of every pair.
  * Returns 0 ok, 1 launch error / unsupported class, 2 nroots >= 12,
- * 3 too many bins. */
+ * 3 more than 4096 bins, 4 nao above 46340. */
 int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
                 int n_dm, int ntask, const int *task_cp, const int *task_bins, int nbins,
                 const int *bra_order, int precision, const double *log_q,

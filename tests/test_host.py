@@ -251,3 +251,18 @@ def test_default_tiles_follow_molecule_size():
     finally:
         lib.BQCset_tile(0, -1, 0, 0)
     assert get(1, 1000, 0, 3000) == 256
+
+
+def test_cartesian_ao_limit_is_refused():
+    """nao_c**2 must fit the kernels' 32-bit element offsets: the host plan
+    refuses more than 46340 Cartesian AOs, and the C entry points return 4
+    before touching the device."""
+    hf.check_nao(46340)
+    with pytest.raises(ValueError, match='46340'):
+        hf.check_nao(46341)
+    from byteqc_amd.lib import load_library
+    lib = load_library()
+    assert lib.BQCbuild_jk(None, None, None, None, 46341, 1, 0, None, None, 10, None, 0,
+                           None, None, 0., None) == 4
+    assert lib.GINTbuild_jk(None, None, None, None, 46341, 1, None, None, 10, 0, 0,
+                            None) == 4

@@ -204,3 +204,52 @@ def test_fp32_everywhere_cc_pvtz_stays_at_fp32_accuracy():
     dj, dk = _report('1 H2O cc-pVTZ all FP32', j1, k1, j0, k0)
     scale = max(abs(j0).max(), abs(k0).max())
     assert dj < 1e-5 * scale and dk < 1e-5 * scale
+
+
+def test_scf_hook_caches_plan_and_matches_oracle():
+    """_get_jk (cuobc/scf/hf.py:230-250): the plan is built once per SCF
+    object and reused; J/K equal the oracle's."""
+    _torch_gpu()
+    from byteqc_amd.scf import hf
+    from byteqc_amd.systems import water_mole, random_density
+    from oracle import oracle as orc
+    mol = water_mole(3, 6.0, 31, 'cc-pvdz')
+
+    class MF:
+        direct_scf_tol = 1e-13
+
+        def __init__(self, mol):
+            self.mol = mol
+
+    mf = MF(mol)
+    dms = [random_density(mol.nao, 15, s) for s in (41, 42)]
+    plans = []
+    for dm in dms:
+        j, k = hf._get_jk(mf, dm=dm)
+        plans.append(mf._opt_gpu)
+        jref, kref = orc.get_jk(mol, dm)
+        dj, dk = _report('_get_jk 3 H2O cc-pVDZ', j, k, jref, kref)
+        assert dj < TOL and dk < TOL
+    assert plans[0] is plans[1]
+    j, none = hf._get_jk(mf, mol, dms[0], with_k=False)
+    assert none is None and numpy.abs(j - orc.get_jk(mol, dms[0])[0]).max() < TOL
+
+
+@pytest.mark.parametrize('group_size', [12, 40])
+def test_group_size_build_matches_oracle(group_size):
+    """Pair groups capped at group_size AOs (_split_l_ctr_groups,
+    cuobc/scf/hf.py:581-606): more, smaller tasks, the same J/K."""
+    _torch_gpu()
+    from byteqc_amd.scf import hf
+    from byteqc_amd.systems import water_mole, random_density
+    from oracle import oracle as orc
+    mol = water_mole(4, 6.5, 37, 'def2-svp')
+    dm = random_density(mol.nao, 20, 43)
+    opt = hf._VHFOpt(mol).build(group_size=group_size)
+    assert len(opt.uniq_l_ctr) > len(hf._VHFOpt(mol).build().uniq_l_ctr)
+    j, k = hf.get_jk(mol, dm, vhfopt=opt)
+    jref, kref = orc.get_jk(mol, dm)
+    dj, dk = _report(f'group_size {group_size}', j, k, jref, kref)
+    assert dj < TOL and dk < TOL
+    j2, k2 = hf.get_jk(mol, dm, group_size=group_size)
+    assert numpy.abs(j2 - j).max() < 1e-12 and numpy.abs(k2 - k).max() < 1e-12
