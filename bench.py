@@ -22,6 +22,12 @@ feeds Poisson arrivals of RHF requests (5-120 waters, def2-SVP) into the
 serving loop (``--qps``, ``--duration``, ``--lanes``) and reports latency
 percentiles.
 
+The default c1/c2 run adds two objects to the line: ``mixed``, the same
+workload timed in mixed precision with its ratio to FP64 and |dE|, max|dJ|,
+max|dK| against the FP64 build, and ``parity_vs_oracle``, the FP64 and mixed
+results against a full CPU-oracle build of the same input (about half a
+minute on 16 host threads at c2; ``--no-mixed``, ``--no-oracle`` skip them).
+
 Prints one JSON line.  ``roofline`` is for the kernel class that takes the
 most time: its algorithmic FLOPs per launch (quartet count x the per-quartet
 FLOP model) over its mean launch time, measured with HIP events on the
@@ -65,18 +71,32 @@ def _threads():
     return int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
 
 
-def cpu_baseline(mol, dm, stride):
-    """Oracle J/K on every stride-th kl pair of every task, scaled to builds/s."""
+def cpu_baseline(mol, dm, stride, stride1=64):
+    """Oracle J/K on every stride-th kl pair of every task, scaled to builds/s,
+    on all host threads; and on one thread over every stride1-th kl pair."""
+    import ctypes
     from oracle import oracle as orc
     plan = orc.build_plan(mol)
     t0 = time.perf_counter()
     _, _, nq = orc.get_jk(mol, dm, plan, return_count=True, kl_stride=stride)
     dt = time.perf_counter() - t0
-    return {'value': 1.0 / (dt * stride), 'unit': 'J/K builds/s', 'cores': _threads(),
+    lib = orc.lib()
+    nthr = lib.oracle_get_threads()
+    lib.oracle_set_threads(ctypes.c_int(1))
+    try:
+        t1 = time.perf_counter()
+        _, _, nq1 = orc.get_jk(mol, dm, plan, return_count=True, kl_stride=stride1)
+        dt1 = time.perf_counter() - t1
+    finally:
+        lib.oracle_set_threads(ctypes.c_int(nthr))
+    return {'value': 1.0 / (dt * stride), 'unit': 'J/K builds/s', 'cores': nthr,
             'kind': 'port',
             'sample': f'oracle on every {stride}th kl pair of every task '
-                      f'({nq} quartets, {dt:.1f} s on {_threads()} threads of {_cpu_model()}); '
-                      f'scaled x{stride} to one build'}
+                      f'({nq} quartets, {dt:.1f} s on {nthr} threads of {_cpu_model()}); '
+                      f'scaled x{stride} to one build',
+            'one_thread': {'value': 1.0 / (dt1 * stride1), 'cores': 1,
+                           'sample': f'every {stride1}th kl pair ({nq1} quartets, {dt1:.1f} s '
+                                     f'on 1 thread); scaled x{stride1}'}}
 
 
 def cpu_baseline_batch(mols, dms, nsample, stride, frac):
@@ -102,12 +122,16 @@ def cpu_baseline_batch(mols, dms, nsample, stride, frac):
 KIND_NAMES = {0: 'jk_quartets', 1: 'jk_rows', 4: 'jk_wave'}
 
 
-def kernel_name(cls, kind):
+def kernel_name(cls, kind, split=False):
     """Template name of a profiled launch: class code li*1000+lj*100+lk*10+ll,
-    kind 0 quartet / 1 row / 4 wavefront kernel, +2 for the FP32 instance."""
+    kind 0 quartet / 1 row / 4 wavefront kernel, +2 for the FP32 instance, +8
+    for the b-side pass of a split row class.  split names the a-side and
+    b-side row kernels of a class together."""
     ls = (cls // 1000, cls // 100 % 10, cls // 10 % 10, cls % 10)
     t = 'float' if kind & 2 else 'double'
-    return '%s<%d,%d,%d,%d,%s>' % (KIND_NAMES[kind & ~2], *ls, t)
+    if split:
+        return ' + '.join('%s<%d,%d,%d,%d,%s,%d>' % (KIND_NAMES[1], *ls, t, r) for r in (1, 2))
+    return '%s<%d,%d,%d,%d,%s>' % (KIND_NAMES[kind & ~10], *ls, t)
 
 
 def measured_traffic(workload, kernel):
@@ -139,16 +163,20 @@ def roofline(recs, steps, workload):
     """Dominant (class, kind) over the records of the timed steps."""
     by_cls = {}
     for r in recs:
-        d = by_cls.setdefault((r['cls'], r['kind']), {'ms': 0.0, 'n': 0, 'flops': 0.0})
+        # the two passes of a split row class (kind 1 and 9) count as one
+        # kernel: their launches together evaluate and digest the class
+        d = by_cls.setdefault((r['cls'], r['kind'] & ~8),
+                              {'ms': 0.0, 'n': 0, 'flops': 0.0, 'split': False})
         d['ms'] += r['ms']
-        d['n'] += 1
+        d['n'] += 0 if r['kind'] & 8 else 1
         d['flops'] += r['flops']
+        d['split'] = d['split'] or bool(r['kind'] & 8)
     (cls, kind), dom = max(by_cls.items(), key=lambda kv: kv[1]['ms'])
     avg_ms = dom['ms'] / dom['n']
     flops_per_launch = dom['flops'] / dom['n']
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     peak = FP32_PEAK_TFLOPS if kind & 2 else FP64_PEAK_TFLOPS
-    name = kernel_name(cls, kind)
+    name = kernel_name(cls, kind, dom['split'])
     kern_ms = sum(v['ms'] for v in by_cls.values()) / steps
     return {'bound': 'valu-fp32' if kind & 2 else 'valu-fp64', 'kernel': name,
             'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
@@ -241,10 +269,62 @@ def run_single(args, torch):
             'max_dJ': float(numpy.abs(j - ref[0]).max()),
             'max_dK': float(numpy.abs(k - ref[1]).max()),
             'fp32_flop_share': flops[1] / max(flops[0] + flops[1], 1.0)}
+    elif not args.no_mixed:
+        out['mixed'] = mixed_leg(args, torch, step, dm, out['ms_per_step'])
     if args.copy_probe:
         copy_probe(torch)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(mol, dm, args.cpu_stride)
+    if not args.no_oracle and args.precision == 'fp64':
+        out['parity_vs_oracle'] = oracle_parity(mol, dm, step)
+    return out
+
+
+def mixed_leg(args, torch, step, dm, fp64_ms):
+    """The same workload in mixed precision (FP32 below a Schwarz x density
+    bound of 1e-7): timed like the FP64 steps, with |dE| and max|dJ|, max|dK|
+    against an FP64 build of the same input."""
+    from byteqc_amd.scf import hf
+    rj, rk = step('fp64')
+    ref = (rj.cpu().numpy(), rk.cpu().numpy())
+    for _ in range(args.warmup):
+        step('mixed')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vj, vk = step('mixed')
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    fl = hf.last_build_flops()
+    j, k = vj.cpu().numpy(), vk.cpu().numpy()
+    out = {'ms_per_step': ms, 'builds_s': 1e3 / ms, 'fp64_ms_per_step': fp64_ms,
+           'speedup_vs_fp64': fp64_ms / ms,
+           'fp32_flop_share': fl[1] / max(fl[0] + fl[1], 1.0),
+           'vs_fp64': {'dE': abs(energy_terms(dm, j, k) - energy_terms(dm, *ref)),
+                       'max_dJ': float(numpy.abs(j - ref[0]).max()),
+                       'max_dK': float(numpy.abs(k - ref[1]).max())}}
+    log(f"mixed: {ms:.1f} ms per build ({out['speedup_vs_fp64']:.3f}x FP64), "
+        f"|dE| {out['vs_fp64']['dE']:.2e} Eh")
+    return out
+
+
+def oracle_parity(mol, dm, step):
+    """FP64 and mixed J/K of the benchmark workload against the CPU oracle's
+    full build (max |dJ|, max |dK|, |dE| with E = 1/2 Tr D(J - K/2))."""
+    from oracle import oracle as orc
+    t0 = time.perf_counter()
+    jref, kref = orc.get_jk(mol, dm)
+    dt = time.perf_counter() - t0
+    eref = energy_terms(dm, jref, kref)
+    out = {'oracle_s': dt, 'oracle_threads': _threads()}
+    for prec in ('fp64', 'mixed'):
+        vj, vk = step(prec)
+        j, k = vj.cpu().numpy(), vk.cpu().numpy()
+        out[prec] = {'max_dJ': float(numpy.abs(j - jref).max()),
+                     'max_dK': float(numpy.abs(k - kref).max()),
+                     'dE': abs(energy_terms(dm, j, k) - eref)}
+    log(f"oracle parity ({dt:.1f} s): FP64 max|dJ| {out['fp64']['max_dJ']:.2e} "
+        f"max|dK| {out['fp64']['max_dK']:.2e}; mixed |dE| {out['mixed']['dE']:.2e} Eh")
     return out
 
 
@@ -358,7 +438,7 @@ def run_serve(args, torch):
     srv.drain()
     srv.close()
     lanes = tuple(int(x) for x in args.lanes.split(',')) if args.lanes else (800, 1600)
-    srv = Server(log=log, lanes=lanes)
+    srv = Server(log=log, lanes=lanes, init_guess=args.guess)
     reqs, wall = run_load(srv, arr, lambda n, sd: water_mole(n, box_for_density(n), sd,
                                                             'def2-svp'), log=log)
     srv.close()
@@ -374,8 +454,8 @@ def run_serve(args, torch):
         'dtype': 'f64+f32', 'data': 'synthetic',
         'config': {'workload': f'c5: RHF requests, Poisson {args.qps} QPS for '
                                f'{args.duration} s, {args.min_water}-{args.max_water} H2O '
-                               'log-uniform, def2-svp, mixed J/K until |dE|<1e-5 then FP64, '
-                               'converged to 1e-8 Eh',
+                               f'log-uniform, def2-svp, {args.guess} initial guess, mixed J/K '
+                               'until |dE|<1e-5 then FP64, converged to 1e-8 Eh',
                    'requests': len(reqs), 'lanes': list(lanes), 'parallelism': 'single GPU'},
         'serving': {'offered_qps': args.qps, 'completed_qps': len(reqs) / wall,
                     'wall_s': wall, 'converged': len(ok), 'requests': len(reqs),
@@ -401,6 +481,8 @@ def main():
     ap.add_argument('--qps', type=float, default=0.5, help='c5: request rate')
     ap.add_argument('--duration', type=float, default=60.0, help='c5: seconds of arrivals')
     ap.add_argument('--lanes', default='', help='c5: AO counts separating the size lanes')
+    ap.add_argument('--guess', default='hcore', choices=('hcore', 'atom'),
+                    help='c5: initial guess of the SCF requests (core Hamiltonian by default)')
     ap.add_argument('--min-water', type=int, default=5)
     ap.add_argument('--max-water', type=int, default=120)
     ap.add_argument('--precision', default='fp64', choices=('fp64', 'mixed'))
@@ -412,9 +494,19 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=4,
                     help='c3: molecules the CPU baseline runs')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--row-split', type=int, default=1, choices=(0, 1),
+                    help='two-pass (a-side / b-side) digestion of the small s/p row classes')
+    ap.add_argument('--persistent', type=int, default=1, choices=(0, 1),
+                    help='persistent work-queue grids for the row and wavefront kernels')
+    ap.add_argument('--no-mixed', action='store_true',
+                    help='skip the mixed-precision leg of the c1/c2/c4 runs')
+    ap.add_argument('--no-oracle', action='store_true',
+                    help='skip the full CPU-oracle parity check of the c1/c2 runs')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run a 512 MiB streaming kernel (HBM counter calibration)')
     args = ap.parse_args()
+    if args.workload == 'c4':
+        args.no_oracle = True  # the CPU oracle needs hours for 128 waters cc-pVTZ
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for --gpus {args.gpus}); '
               'multi-GPU sharding of the quartet list is not part of this build.',
@@ -423,7 +515,9 @@ def main():
 
     import torch
     from byteqc_amd.lib import load_library
-    load_library()
+    lib = load_library()
+    lib.BQCset_row_split(-1, args.row_split)
+    lib.BQCset_launch_mode(args.persistent)
     torch.cuda.set_device(0)
     if args.workload == 'c3':
         out = run_batch(args, torch)

@@ -20,7 +20,7 @@ ARCH = os.environ.get('BQC_OFFLOAD_ARCH', 'gfx950')
 
 SOURCES = ['jk.hip', 'jk_bra_00.hip', 'jk_bra_10.hip', 'jk_bra_11.hip', 'jk_bra_20.hip',
            'jk_bra_21.hip', 'jk_bra_22.hip', 'jk_bra_30.hip', 'jk_bra_31.hip', 'jk_bra_32.hip',
-           'jk_bra_33.hip', 'schwarz.hip', 'int1e.hip', 'dmcond.hip', 'fill.hip', 'fill_bra_a.hip',
+           'jk_bra_33.hip', 'jk_general.hip', 'schwarz.hip', 'int1e.hip', 'dmcond.hip', 'fill.hip', 'fill_bra_a.hip',
            'fill_bra_b.hip', 'fill_bra_c.hip', 'fill_bra_d.hip', 'pair_data.cpp', 'planner.cpp']
 HIP_FLAGS = ['-O3', '-fPIC', '-std=c++17', f'--offload-arch={ARCH}', '-munsafe-fp-atomics',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',

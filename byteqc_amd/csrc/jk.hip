@@ -111,6 +111,7 @@ struct TaskCtx {
     const int *bins_ij, *bins_kl;
     int li, lj, lk, ll, pij, pkl;
     bool swap, diag;
+    bool gen;            // a g or h shell: the run-time-class kernel, FP64
     double log_cut;      // -inf: FP64 only
     const double *lq;    // host log q per pair (plan order) or NULL
     double qflops;       // FLOPs of one quartet of this class
@@ -138,17 +139,18 @@ static int task_init(TaskCtx &c, BasisProdCache *bp, const int *bins_ij, const i
     c.lk = tkl->l_bra;
     c.ll = tkl->l_ket;
     if ((c.li + c.lj + c.lk + c.ll) / 2 + 1 >= 12) return 2;
-    if (c.li > BQC_LMAX || c.lj > BQC_LMAX || c.lk > BQC_LMAX || c.ll > BQC_LMAX ||
-        c.lj > c.li || c.ll > c.lk)
+    if (c.li > BQC_LMAX_GEN || c.lj > BQC_LMAX_GEN || c.lk > BQC_LMAX_GEN ||
+        c.ll > BQC_LMAX_GEN || c.lj > c.li || c.ll > c.lk)
         return 1;
+    c.gen = c.li > BQC_LMAX || c.lk > BQC_LMAX;
     if (nbins > BQC_MAX_BINS) return 3;
     if (bins_ij[0] != 0 || bins_kl[0] != 0) return 1;
     c.pij = pair_id(c.li, c.lj);
     c.pkl = pair_id(c.lk, c.ll);
-    c.swap = c.pij < c.pkl;
+    c.swap = !c.gen && c.pij < c.pkl;
     c.diag = cp_ij == cp_kl;
     c.lq = lq;
-    c.log_cut = (precision == 1 && lq && dmax > 0) ? log(fp32_cut / dmax) : -1e300;
+    c.log_cut = (precision == 1 && lq && dmax > 0 && !c.gen) ? log(fp32_cut / dmax) : -1e300;
     c.qflops = quartet_flops(c.li, c.lj, c.lk, c.ll, tij->nprim_12 * tkl->nprim_12);
     return 0;
 }
@@ -178,18 +180,25 @@ static int eval_cls(const TaskCtx &c) {
                   : c.li * 1000 + c.lj * 100 + c.lk * 10 + c.ll;
 }
 
+// Classes with a g or h shell (jk_general.hip).
+int bqc_launch_general(const JKArgs &jk, const PairData &pd, const TaskArgs &ta, int li, int lj,
+                       int lk, int ll, hipStream_t stream);
+
 // Per-lane scratch (private segment) limit for a launch; the runtime aborts
 // the process when it cannot back a kernel's scratch, so oversized kernels
 // are refused here instead.
 // A kernel whose static LDS exceeds the device's per-workgroup limit is
 // refused the same way.  Answers are cached per kernel.
 #define BQC_MAX_SCRATCH_BYTES 32768
+#define BQC_OK_CACHE 1024
 static bool scratch_ok(const void *kern) {
     static std::mutex mu;
-    static std::map<const void *, bool> known;
+    static const void *keys[BQC_OK_CACHE];
+    static bool vals[BQC_OK_CACHE];
+    static int nkeys = 0;
     std::lock_guard<std::mutex> lock(mu);
-    auto it = known.find(kern);
-    if (it != known.end()) return it->second;
+    for (int i = 0; i < nkeys; ++i)
+        if (keys[i] == kern) return vals[i];
     hipFuncAttributes attr;
     bool ok = hipFuncGetAttributes(&attr, kern) == hipSuccess &&
               attr.localSizeBytes <= BQC_MAX_SCRATCH_BYTES;
@@ -198,8 +207,51 @@ static bool scratch_ok(const void *kern) {
         hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
             hipSuccess)
         ok = attr.sharedSizeBytes <= (size_t)lds_max;
-    known[kern] = ok;
+    if (nkeys < BQC_OK_CACHE) {
+        keys[nkeys] = kern;
+        vals[nkeys++] = ok;
+    }
     return ok;
+}
+
+// Builds through the allocation-free entry point set this (per thread):
+// their launches skip the optional per-launch timing, whose records live on
+// the heap.
+static thread_local bool t_noalloc = false;
+static bool prof_on() { return g_prof && !t_noalloc; }
+
+// Launch shape of the row and wave kernels: persistent grids (workgroups
+// that stay resident and pull work items from a counter) or one workgroup
+// per item (four for the row kernels).  Set by BQCset_launch_mode.
+static bool g_persistent = true;
+
+// Workgroups of kern (nthr lanes) resident on the whole device at once:
+// occupancy per CU times the CU count.  Cached per kernel and device in a
+// fixed table (no heap, so the allocation-free entry may use it).
+#define BQC_OCC_CACHE 1024
+static unsigned resident_blocks(const void *kern, int nthr) {
+    static std::mutex mu;
+    static const void *keys[BQC_OCC_CACHE];
+    static int devs[BQC_OCC_CACHE];
+    static unsigned vals[BQC_OCC_CACHE];
+    static int nkeys = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < nkeys; ++i)
+        if (keys[i] == kern && devs[i] == dev) return vals[i];
+    int per_cu = 0, ncu = 0;
+    unsigned v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nthr, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        per_cu > 0 && ncu > 0)
+        v = (unsigned)per_cu * (unsigned)ncu;
+    if (nkeys < BQC_OCC_CACHE) {
+        keys[nkeys] = kern;
+        devs[nkeys] = dev;
+        vals[nkeys++] = v;
+    }
+    return v;
 }
 
 // Generic kernels (one lane per quartet): rectangle kl-bin b x ij in
@@ -233,6 +285,16 @@ static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &p
         if (t.nbins == 0) return 0;
         const double lflops = nq[prec] * c.qflops;
         flops[prec] += lflops;
+        if (c.gen) {
+            LaunchRec rec;
+            if (prof_on()) record_launch(rec, eval_cls(c), 0, lflops, nq[prec], stream);
+            int e = bqc_launch_general(jk, pd, t, c.li, c.lj, c.lk, c.ll, stream);
+            if (prof_on()) keep_launch(rec, stream);
+            if (e)
+                fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d)\n", c.li, c.lj, c.lk,
+                        c.ll);
+            return e;
+        }
         JKKernel kern = c.swap ? g_kernels[prec][c.pkl][c.pij] : g_kernels[prec][c.pij][c.pkl];
         if (!kern) return 1;
         if (!scratch_ok((const void *)kern)) {
@@ -243,9 +305,9 @@ static int launch_quartets(const TaskCtx &c, const JKArgs &jk, const PairData &p
         long long nblk = (t.pre[t.nbins] + BQC_THREADS - 1) / BQC_THREADS;
         if (nblk > 0x7fffffffLL) return 1;
         LaunchRec rec;
-        if (g_prof) record_launch(rec, eval_cls(c), 2 * prec, lflops, nq[prec], stream);
+        if (prof_on()) record_launch(rec, eval_cls(c), 2 * prec, lflops, nq[prec], stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(BQC_THREADS), 0, stream, jk, pd, t);
-        if (g_prof) keep_launch(rec, stream);
+        if (prof_on()) keep_launch(rec, stream);
         hipError_t err = hipGetLastError();
         if (err != hipSuccess) {
             fprintf(stderr, "bqcjk: launch failed for (%d%d|%d%d): %s\n", c.li, c.lj, c.lk, c.ll,
@@ -344,35 +406,108 @@ static int tile_for(int kind, int cls, int prec, int nao) {
     return big ? 256 : 32;
 }
 
-struct Group {
+// Orders of the pair lists of a plan by first shell (which = 0; for the
+// reference-ABI entry GINTbuild_jk, whose caller has no such array) and by
+// second shell (which = 1; the b-side lists of split row classes): per pair
+// type the plan positions sorted by that shell, stable (a counting sort of a
+// row of the host bas_pair2shls, which the cache aliases).  Built per pair
+// type on first use and kept per cache until BQCdel_basis_prod_clone or
+// GINTdel_basis_prod forgets it.
+struct PairOrder {
+    const int *pair2shls = nullptr;
+    int npairs = 0, ncptype = 0;
+    std::vector<int> order[2];
+    std::vector<char> done[2];
+};
+static std::mutex g_order_mu;
+static std::map<const BasisProdCache *, PairOrder> g_orders;
+
+static const int *shell_order(const BasisProdCache *bp, int which, int cp_a, int cp_b) {
+    const int *locs = bp->bas_pairs_locs;
+    const int npairs = locs[bp->ncptype];
+    std::lock_guard<std::mutex> lock(g_order_mu);
+    PairOrder &po = g_orders[bp];
+    if (po.pair2shls != bp->bas_pair2shls || po.npairs != npairs ||
+        po.ncptype != bp->ncptype) {
+        po.pair2shls = bp->bas_pair2shls;
+        po.npairs = npairs;
+        po.ncptype = bp->ncptype;
+        for (int w = 0; w < 2; ++w) {
+            po.order[w].assign(npairs > 0 ? npairs : 1, 0);
+            po.done[w].assign(bp->ncptype, 0);
+        }
+    }
+    const int cps[2] = {cp_a, cp_b};
+    for (int cp : cps) {
+        if (po.done[which][cp]) continue;
+        const int p0 = locs[cp], n = locs[cp + 1] - locs[cp];
+        const int *key = bp->bas_pair2shls + (which ? npairs : 0) + p0;
+        int lo = 0, hi = -1;
+        for (int k = 0; k < n; ++k) {
+            if (k == 0 || key[k] < lo) lo = key[k];
+            if (key[k] > hi) hi = key[k];
+        }
+        std::vector<int> cnt(n > 0 ? hi - lo + 2 : 1, 0);
+        for (int k = 0; k < n; ++k) ++cnt[key[k] - lo + 1];
+        for (size_t v = 1; v < cnt.size(); ++v) cnt[v] += cnt[v - 1];
+        for (int k = 0; k < n; ++k) po.order[which][p0 + cnt[key[k] - lo]++] = k;
+        po.done[which][cp] = 1;
+    }
+    return po.order[which].data();
+}
+
+extern "C" void bqc_forget_pair_order(const BasisProdCache *bp) {
+    std::lock_guard<std::mutex> lock(g_order_mu);
+    g_orders.erase(bp);
+}
+
+// Split row classes (a-side and b-side passes) by class code; -1 entries are
+// unset (default on).  Set by BQCset_row_split.
+#define BQC_SPLIT_MAX 64
+static int g_split_cls[BQC_SPLIT_MAX], g_split_val[BQC_SPLIT_MAX], g_split_n = 0;
+static int g_split_all = 1;
+static bool split_on(int cls) {
+    for (int i = 0; i < g_split_n; ++i)
+        if (g_split_cls[i] == cls) return g_split_val[i] != 0;
+    return g_split_all != 0;
+}
+
+template <class T> using StdVec = std::vector<T>;
+
+template <template <class> class V>
+struct GroupT {
     RowKernel kern;
     int wave, cls, prec;
     double flops, nquart;
-    std::vector<int> items;  // int4 per item
+    V<int> items;     // int4 per item
+    int role_b = 0;   // b-side pass of a split row class
 };
 
-struct BuildPlan {
-    std::vector<TaskDesc> tasks;
-    std::vector<size_t> perm_p_off, perm_q_off, bins_off;  // in ints
-    std::vector<int> ints;                       // list orders
-    std::vector<Group> groups;
+template <template <class> class V>
+struct BuildPlanT {
+    V<TaskDesc> tasks;
+    V<size_t> perm_p_off, perm_q_off, bins_off;  // in ints
+    V<int> ints;                                 // list orders and bins
+    V<GroupT<V>> groups;
     double flops[2] = {0, 0};
 
-    Group &group(RowKernel k, int wave, int cls, int prec) {
-        for (Group &g : groups)
+    GroupT<V> &group(RowKernel k, int wave, int cls, int prec) {
+        for (GroupT<V> &g : groups)
             if (g.kern == k) return g;
-        groups.push_back(Group{k, wave, cls, prec, 0, 0, {}});
+        groups.push_back(GroupT<V>{k, wave, cls, prec, 0, 0, {}, 0});
         return groups.back();
     }
 };
+using BuildPlan = BuildPlanT<StdVec>;
 
 // List order of one side: plan positions [0, bins[nbins]) grouped by bin,
 // inside a bin by first shell (bra_order gives the plan positions of the
 // pair type sorted by first shell, then second shell).
+template <class IntVec>
 static void side_order(const int *bins, int nbins, const int *order, int npairs_cp,
-                       std::vector<int> &out) {
+                       IntVec &out) {
     const int n = bins[nbins];
-    std::vector<int> cursor(bins, bins + nbins);
+    IntVec cursor(bins, bins + nbins);
     size_t base = out.size();
     out.resize(base + n);
     for (int k = 0; k < npairs_cp; ++k) {
@@ -387,14 +522,15 @@ static void side_order(const int *bins, int nbins, const int *order, int npairs_
 // class when there is one within the scratch and LDS limits, else the wave
 // kernel.  Returns -1 if a precision has neither (the generic kernels take
 // the task).
-static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, const JKArgs &jk,
-                     const PairData &pd) {
-    if (!bra_order) return -1;
+template <template <class> class V>
+static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
+                     const JKArgs &jk, const PairData &pd) {
+    if (!bra_order || c.gen) return -1;
     RowKernel k[2];
     int wv[2];
     bool fp32 = true;
     for (int prec = 0; prec < 2; ++prec) {
-        RowKernel r = c.swap ? g_rows[prec][c.pkl][c.pij] : g_rows[prec][c.pij][c.pkl];
+        RowKernel r = c.swap ? g_rows[0][prec][c.pkl][c.pij] : g_rows[0][prec][c.pij][c.pkl];
         RowKernel w = c.swap ? g_waves[prec][c.pkl][c.pij] : g_waves[prec][c.pij][c.pkl];
         if (r && scratch_ok((const void *)r)) {
             k[prec] = r;
@@ -412,6 +548,19 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
     }
     // a row class without an FP32 instance runs all its quartets in FP64
     const double log_cut = fp32 ? c.log_cut : -1e300;
+    // split row class: a-side and b-side passes (FP64 builds of the class only)
+    RowKernel ks[2] = {nullptr, nullptr};
+    const int *order2 = nullptr;
+    if (!wv[0] && !t_noalloc && log_cut <= -1e299 && split_on(eval_cls(c)) &&
+        c.bp->bas_pair2shls) {
+        for (int role = 1; role <= 2; ++role)
+            ks[role - 1] = c.swap ? g_rows[role][0][c.pkl][c.pij] : g_rows[role][0][c.pij][c.pkl];
+        if (ks[0] && ks[1] && scratch_ok((const void *)ks[0]) && scratch_ok((const void *)ks[1])) {
+            const int cp_p = c.swap ? c.cp_kl : c.cp_ij;
+            order2 = shell_order(c.bp, 1, cp_p, cp_p);
+            k[0] = ks[0];
+        }
+    }
     BasisProdCache *bp = c.bp;
     const int nbins = c.nbins;
     const int *pbins = c.swap ? c.bins_kl : c.bins_ij;
@@ -467,7 +616,7 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
     bp_.perm_p_off.push_back(off_p);
     bp_.perm_q_off.push_back(off_q);
     const int cls = eval_cls(c);
-    Group *g[2] = {&bp_.group(k[0], wv[0], cls, 0), nullptr};
+    GroupT<V> *g[2] = {&bp_.group(k[0], wv[0], cls, 0), nullptr};
     g[1] = &bp_.group(k[1], wv[1], cls, 1);
     g[0] = &bp_.group(k[0], wv[0], cls, 0);  // re-fetch: group() may reallocate
     for (int prec = 0; prec < 2; ++prec) {
@@ -494,7 +643,7 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
     for (int prec = 0; prec < 2; ++prec) {
         const int ck = tile_for(wv[prec] ? 4 : 1, cls, prec, jk.nao);
         const int step = wv[prec] ? 1 : 64;
-        std::vector<int> &it = g[prec]->items;
+        V<int> &it = g[prec]->items;
         for (int p = 0; p < np; p += step) {
             int qmax, qsplit;
             q_limits(p, qmax, qsplit);
@@ -514,23 +663,54 @@ static int plan_task(BuildPlan &bp_, const TaskCtx &c, const int *bra_order, con
             }
         }
     }
+    if (order2) {
+        // b-side pass: the same pairs sorted by second shell inside each bin,
+        // the same Q lists, bins and work items
+        const int ti2 = (int)bp_.tasks.size();
+        const TaskDesc td2 = bp_.tasks[ti];
+        const size_t bins2 = bp_.bins_off[ti];
+        bp_.tasks.push_back(td2);
+        bp_.bins_off.push_back(bins2);
+        const size_t permq2 = bp_.perm_q_off[ti];
+        bp_.perm_p_off.push_back(bp_.ints.size());
+        bp_.perm_q_off.push_back(permq2);
+        side_order(pbins, nbins, order2 + locs[cp_p], locs[cp_p + 1] - locs[cp_p], bp_.ints);
+        GroupT<V> &g1 = bp_.group(k[0], 0, cls, 0);
+        const size_t n1 = g1.items.size();
+        size_t first = n1;
+        while (first >= 4 && g1.items[first - 4] == ti) first -= 4;
+        GroupT<V> &g2 = bp_.group(ks[1], 0, cls, 0);
+        g2.role_b = 1;
+        GroupT<V> &g1b = bp_.group(k[0], 0, cls, 0);  // group() may reallocate
+        for (size_t e = first; e < n1; e += 4) {
+            g2.items.push_back(ti2);
+            g2.items.push_back(g1b.items[e + 1]);
+            g2.items.push_back(g1b.items[e + 2]);
+            g2.items.push_back(g1b.items[e + 3]);
+        }
+    }
     return 0;
 }
 
 // Device image of a BuildPlan: [TaskDesc array][list orders][items].
-struct Staged {
-    size_t bytes = 0, off_perm = 0;
-    std::vector<size_t> off_items;  // per group
+template <template <class> class V>
+struct StagedT {
+    size_t bytes = 0, off_perm = 0, off_tickets = 0;
+    V<size_t> off_items;  // per group
 };
+using Staged = StagedT<StdVec>;
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-static Staged stage_layout(const BuildPlan &p) {
-    Staged st;
+template <template <class> class V>
+static StagedT<V> stage_layout(const BuildPlanT<V> &p) {
+    StagedT<V> st;
     size_t off = align16(p.tasks.size() * sizeof(TaskDesc));
     st.off_perm = off;
     off = align16(off + p.ints.size() * sizeof(int));
-    for (const Group &g : p.groups) {
+    st.off_tickets = off;  // one work counter per group, zero in the image
+    off = align16(off + p.groups.size() * sizeof(int));
+    for (const GroupT<V> &g : p.groups) {
         st.off_items.push_back(off);
         off = align16(off + g.items.size() * sizeof(int));
     }
@@ -539,7 +719,8 @@ static Staged stage_layout(const BuildPlan &p) {
 }
 
 // Writes the image for device address dev into host buffer host.
-static void stage_fill(BuildPlan &p, const Staged &st, char *dev, char *host) {
+template <template <class> class V>
+static void stage_fill(BuildPlanT<V> &p, const StagedT<V> &st, char *dev, char *host) {
     int *perm_dev = (int *)(dev + st.off_perm);
     for (size_t t = 0; t < p.tasks.size(); ++t) {
         p.tasks[t].perm_p = perm_dev + p.perm_p_off[t];
@@ -549,24 +730,39 @@ static void stage_fill(BuildPlan &p, const Staged &st, char *dev, char *host) {
     }
     memcpy(host, p.tasks.data(), p.tasks.size() * sizeof(TaskDesc));
     memcpy(host + st.off_perm, p.ints.data(), p.ints.size() * sizeof(int));
+    memset(host + st.off_tickets, 0, p.groups.size() * sizeof(int));
     for (size_t gi = 0; gi < p.groups.size(); ++gi)
         memcpy(host + st.off_items[gi], p.groups[gi].items.data(),
                p.groups[gi].items.size() * sizeof(int));
 }
 
-static int launch_groups(const BuildPlan &p, const Staged &st, char *dev, hipStream_t stream) {
+template <template <class> class V>
+static int launch_groups(const BuildPlanT<V> &p, const StagedT<V> &st, char *dev,
+                         hipStream_t stream) {
     const TaskDesc *tasks = (const TaskDesc *)dev;
     for (size_t gi = 0; gi < p.groups.size(); ++gi) {
-        const Group &g = p.groups[gi];
+        const GroupT<V> &g = p.groups[gi];
         int nitems = (int)g.items.size() / 4;
         if (!nitems) continue;
         const int4 *items = (const int4 *)(dev + st.off_items[gi]);
         unsigned nblk = g.wave ? (unsigned)nitems : (unsigned)((nitems + 3) / 4);
         unsigned nthr = g.wave ? 64 : 256;
+        int *ticket = nullptr;
+        if (g_persistent) {
+            // as many workgroups as stay resident at once; they drain the list
+            const unsigned cap = resident_blocks((const void *)g.kern, (int)nthr);
+            if (cap > 0 && cap < nblk) {
+                nblk = cap;
+                ticket = (int *)(dev + st.off_tickets) + gi;
+            }
+        }
         LaunchRec rec;
-        if (g_prof) record_launch(rec, g.cls, (g.wave ? 4 : 1) + 2 * g.prec, g.flops, g.nquart, stream);
-        hipLaunchKernelGGL(g.kern, dim3(nblk), dim3(nthr), 0, stream, tasks, items, nitems);
-        if (g_prof) keep_launch(rec, stream);
+        if (prof_on())
+            record_launch(rec, g.cls, (g.wave ? 4 : 1) + 2 * g.prec + 8 * g.role_b, g.flops,
+                          g.nquart, stream);
+        hipLaunchKernelGGL(g.kern, dim3(nblk), dim3(nthr), 0, stream, tasks, items, nitems,
+                           ticket);
+        if (prof_on()) keep_launch(rec, stream);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             fprintf(stderr, "bqcjk: kernel launch failed: %s\n", hipGetErrorString(e));
@@ -664,7 +860,8 @@ struct Deferred {
 
 // Plans all tasks of all molecules.  Classes without row/wave kernels go to
 // the generic kernels: launched right away, or queued in *deferred.
-static int plan_molecules(BuildPlan &plan, const MolArgs *m, int nmol, int nbins,
+template <template <class> class V>
+static int plan_molecules(BuildPlanT<V> &plan, const MolArgs *m, int nmol, int nbins,
                           int precision, double fp32_cut, hipStream_t stream,
                           std::vector<Deferred> *deferred = nullptr) {
     if (precision != 0 && precision != 1) return 1;
@@ -695,57 +892,6 @@ static int plan_molecules(BuildPlan &plan, const MolArgs *m, int nmol, int nbins
     return 0;
 }
 
-// First-shell order of the pair lists of a plan, for the reference-ABI
-// entry GINTbuild_jk (whose caller has no such array): per pair type the
-// plan positions sorted by first shell, stable (a counting sort of the
-// host bra row of bas_pair2shls, which the cache aliases).  Built per pair
-// type on first use and kept per cache until BQCdel_basis_prod_clone or
-// GINTdel_basis_prod forgets it.
-struct PairOrder {
-    const int *pair2shls = nullptr;
-    int npairs = 0, ncptype = 0;
-    std::vector<int> order;
-    std::vector<char> done;
-};
-static std::mutex g_order_mu;
-static std::map<const BasisProdCache *, PairOrder> g_orders;
-
-static const int *first_shell_order(const BasisProdCache *bp, int cp_a, int cp_b) {
-    const int *locs = bp->bas_pairs_locs;
-    const int npairs = locs[bp->ncptype];
-    std::lock_guard<std::mutex> lock(g_order_mu);
-    PairOrder &po = g_orders[bp];
-    if (po.pair2shls != bp->bas_pair2shls || po.npairs != npairs ||
-        po.ncptype != bp->ncptype) {
-        po.pair2shls = bp->bas_pair2shls;
-        po.npairs = npairs;
-        po.ncptype = bp->ncptype;
-        po.order.assign(npairs > 0 ? npairs : 1, 0);
-        po.done.assign(bp->ncptype, 0);
-    }
-    const int cps[2] = {cp_a, cp_b};
-    for (int cp : cps) {
-        if (po.done[cp]) continue;
-        const int p0 = locs[cp], n = locs[cp + 1] - locs[cp];
-        const int *bra = bp->bas_pair2shls + p0;
-        int lo = 0, hi = -1;
-        for (int k = 0; k < n; ++k) {
-            if (k == 0 || bra[k] < lo) lo = bra[k];
-            if (bra[k] > hi) hi = bra[k];
-        }
-        std::vector<int> cnt(n > 0 ? hi - lo + 2 : 1, 0);
-        for (int k = 0; k < n; ++k) ++cnt[bra[k] - lo + 1];
-        for (size_t v = 1; v < cnt.size(); ++v) cnt[v] += cnt[v - 1];
-        for (int k = 0; k < n; ++k) po.order[p0 + cnt[bra[k] - lo]++] = k;
-        po.done[cp] = 1;
-    }
-    return po.order.data();
-}
-
-extern "C" void bqc_forget_pair_order(const BasisProdCache *bp) {
-    std::lock_guard<std::mutex> lock(g_order_mu);
-    g_orders.erase(bp);
-}
 
 extern "C" {
 
@@ -764,7 +910,7 @@ int GINTbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, in
                         nullptr, 0, 0, 0);
     if (err) return err;
     const int *order = bpcache->bas_pair2shls
-                           ? first_shell_order(bpcache, cp_ij_id, cp_kl_id)
+                           ? shell_order(bpcache, 0, cp_ij_id, cp_kl_id)
                            : nullptr;
     BuildPlan plan;
     err = plan_task(plan, c, order, jk, pd);
@@ -805,6 +951,11 @@ int BQCbuild_jk_batch(int nmol, BasisProdCache **bps, double **vj, double **vk, 
     g_last_flops[0] = plan.flops[0];
     g_last_flops[1] = plan.flops[1];
     return err;
+}
+
+__global__ void zero_ints(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // A captured batch: its own device image and the instantiated graph.
@@ -853,7 +1004,13 @@ void *BQCgraph_capture(int nmol, BasisProdCache **bps, double **vj, double **vk,
             if (!err) err = launch_quartets(d.c, d.jk, d.pd, cs, fl);
         plan.flops[0] += fl[0];
         plan.flops[1] += fl[1];
-        if (!err && !plan.groups.empty()) err = launch_groups(plan, st, g->dev, cs);
+        if (!err && !plan.groups.empty()) {
+            // replays start from zero work counters (a kernel, not a memset)
+            const int ng = (int)plan.groups.size();
+            hipLaunchKernelGGL(zero_ints, dim3((ng + 255) / 256), dim3(256), 0, cs,
+                               (int *)(g->dev + st.off_tickets), ng);
+            err = launch_groups(plan, st, g->dev, cs);
+        }
         hipGraph_t graph = nullptr;
         hipError_t ce = hipStreamEndCapture(cs, &graph);
         if (ce != hipSuccess || !graph) err = 1;
@@ -915,6 +1072,30 @@ int BQCset_tile(int kind, int cls, int prec, int ck) {
             return 0;
         }
     g_ck_override.push_back({key, ck});
+    return 0;
+}
+
+int BQCset_row_split(int cls, int on) {
+    if (on != 0 && on != 1) return 1;
+    if (cls < 0) {
+        g_split_all = on;
+        g_split_n = 0;
+        return 0;
+    }
+    for (int i = 0; i < g_split_n; ++i)
+        if (g_split_cls[i] == cls) {
+            g_split_val[i] = on;
+            return 0;
+        }
+    if (g_split_n >= BQC_SPLIT_MAX) return 1;
+    g_split_cls[g_split_n] = cls;
+    g_split_val[g_split_n++] = on;
+    return 0;
+}
+
+int BQCset_launch_mode(int persistent) {
+    if (persistent != 0 && persistent != 1) return 1;
+    g_persistent = persistent == 1;
     return 0;
 }
 

@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(BQC_THREADS)
 
 
 typedef void (*JKKernel)(JKArgs, PairData, TaskArgs);
-#define BQC_LMAX 3
+#define BQC_LMAX 3      // classes with compiled-in shapes (row, wave, quartet kernels)
+#define BQC_LMAX_GEN 5  // g and h shells: the run-time-class kernel (jk_general.hip)
 #define BQC_NPAIRID ((BQC_LMAX + 1) * (BQC_LMAX + 2) / 2)
 // [precision: 0 FP64, 1 FP32][bra class][ket class]
 typedef JKKernel JKKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];

@@ -17,6 +17,16 @@
 //   K(a,d) += g D(b,c)    segmented sum over runs of equal a at every step
 //   K(b,c) += g D(a,d)    lane register accumulator over a run of equal c
 //   K(b,d) += g D(a,c)    one atomic per lane and component (smallest block)
+//
+// The per-lane K(b,c), K(b,d) atomics dominate the atomic requests of the
+// small s/p classes.  For those classes every quartet can instead be
+// evaluated twice: by the a-side instance (ROLE 1: J(a,b), J(c,d), K(a,c),
+// K(a,d)) over P lists sorted by a, and by the b-side instance (ROLE 2:
+// K(b,c), K(b,d), as register sums over runs of equal c flushed by segmented
+// sums over runs of equal b, and per-step segmented sums over runs of equal
+// b) over the same pairs sorted by b.  The two instances together add the
+// same six targets as ROLE 0 with twice the integral work and several times
+// fewer atomic requests.
 #pragma once
 #include "jk_kernel.h"
 
@@ -120,10 +130,14 @@ constexpr int row_pack_max() {
                 max2(pack_nv(NA * NC), pack_nv(NB * NC)));
 }
 
-// Work item: (task, chunk of 64 P pairs, first and end Q position).
-template <int LA, int LB, int LC, int LD, typename T>
+// Work item: (task, chunk of 64 P pairs, first and end Q position).  With
+// ticket == NULL wavefront w of block b takes item 4b + w; otherwise the grid
+// is persistent and the wavefronts pull items from the counter *ticket
+// (zero before the launch).
+template <int LA, int LB, int LC, int LD, typename T, int ROLE = 0>
 __global__ void __launch_bounds__(256)
-    jk_rows(const TaskDesc *__restrict__ tasks, const int4 *__restrict__ items, int nitems) {
+    jk_rows(const TaskDesc *__restrict__ tasks, const int4 *__restrict__ items, int nitems,
+            int *ticket) {
     using C = Cls<LA, LB, LC, LD>;
     constexpr int NA = C::NFI, NB = C::NFJ, NC = C::NFK, ND = C::NFL;
     // Rys interpolation table of this class's root count staged in LDS
@@ -145,291 +159,322 @@ __global__ void __launch_bounds__(256)
     double *const sv = pk_val + (threadIdx.x >> 6) * 64 * (PK > 0 ? PK : 1);
     int *const so = pk_off + (threadIdx.x >> 6) * 64 * (PK > 0 ? PK : 1);
     const int lane = threadIdx.x & 63;
-    const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (item >= nitems) return;
-    const int4 it = items[item];
-    const TaskDesc &tk = tasks[__builtin_amdgcn_readfirstlane(it.x)];
-    const PairData &pd = tk.pd;
-    const JKArgs jk = {tk.vj, tk.vk, tk.dm, tk.nao, tk.n_dm};
-    const int chunk = __builtin_amdgcn_readfirstlane(it.y);
-    const int q0 = __builtin_amdgcn_readfirstlane(it.z);
-    const int qe = __builtin_amdgcn_readfirstlane(it.w);
-    const int pidx = chunk * 64 + lane;
-    const bool pvalid = pidx < tk.np;
-    // bin of this P pair -> end of its Q prefix
-    int qlim = 0, ppos = 0;
-    if (pvalid) {
-        int s = 0;
-        while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
-        qlim = tk.qbins[tk.nbins - s];
-        ppos = glb(tk.perm_p)[pidx];
-    }
-    const int qend_item = min(qe, tk.qbins[tk.nbins]);
-    int qlim_max = qlim;
+    auto run_item = [&](const int item) {
+        const int4 it = items[item];
+        const TaskDesc &tk = tasks[__builtin_amdgcn_readfirstlane(it.x)];
+        const PairData &pd = tk.pd;
+        const JKArgs jk = {tk.vj, tk.vk, tk.dm, tk.nao, tk.n_dm};
+        const int chunk = __builtin_amdgcn_readfirstlane(it.y);
+        const int q0 = __builtin_amdgcn_readfirstlane(it.z);
+        const int qe = __builtin_amdgcn_readfirstlane(it.w);
+        const int pidx = chunk * 64 + lane;
+        const bool pvalid = pidx < tk.np;
+        // bin of this P pair -> end of its Q prefix
+        int qlim = 0, ppos = 0;
+        if (pvalid) {
+            int s = 0;
+            while (s < tk.nbins && pidx >= tk.pbins[s + 1]) ++s;
+            qlim = tk.qbins[tk.nbins - s];
+            ppos = glb(tk.perm_p)[pidx];
+        }
+        const int qend_item = min(qe, tk.qbins[tk.nbins]);
+        int qlim_max = qlim;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) qlim_max = max(qlim_max, __shfl_xor(qlim_max, d, 64));
-    const int q1 = min(qend_item, __builtin_amdgcn_readfirstlane(qlim_max));
-    if (q1 <= q0) return;
+        for (int d = 32; d >= 1; d >>= 1) qlim_max = max(qlim_max, __shfl_xor(qlim_max, d, 64));
+        const int q1 = min(qend_item, __builtin_amdgcn_readfirstlane(qlim_max));
+        if (q1 <= q0) return;
 
-    // per-lane (P side) and wave-uniform (Q side) views of the plan data
-    const gptr<int> lbra = glb(pd.bra), lket = glb(pd.ket), lao = glb(pd.ao_loc);
-    const gptr<double> lbx = glb(pd.bas_x), lby = glb(pd.bas_y), lbz = glb(pd.bas_z);
-    const cptr<int> ubra = cst(pd.bra), uket = cst(pd.ket), uao = cst(pd.ao_loc);
-    const cptr<int> uperm_q = cst(tk.perm_q);
-    const cptr<double> ubx = cst(pd.bas_x), uby = cst(pd.bas_y), ubz = cst(pd.bas_z);
-    const int gp = tk.bas_p0 + ppos;
-    const int sa = pvalid ? lbra[gp] : -1;
-    const int sb = pvalid ? lket[gp] : 0;
-    const int pp0 = tk.prim_p0 + ppos * tk.nprim_p;
-    const int a0 = pvalid ? lao[sa] : 0;
-    const int b0 = pvalid ? lao[sb] : 0;
-    double ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
-    double p_first[5] = {1, 0, 0, 0, 0};
-    if (pvalid) {
-        ra[0] = lbx[sa]; ra[1] = lby[sa]; ra[2] = lbz[sa];
-        rb[0] = lbx[sb]; rb[1] = lby[sb]; rb[2] = lbz[sb];
-        p_first[0] = glb(pd.a12)[pp0]; p_first[1] = glb(pd.e12)[pp0];
-        p_first[2] = glb(pd.x12)[pp0]; p_first[3] = glb(pd.y12)[pp0];
-        p_first[4] = glb(pd.z12)[pp0];
-    }
-    // runs of equal first shell a among the 64 lanes
-    const int key_next = __shfl_down(sa, 1, 64);
-    const bool tail = lane == 63 || key_next != sa;
-    const int key_prev = __shfl_up(sa, 1, 64);
-    const unsigned long long heads = __ballot(lane == 0 || key_prev != sa);
-    const int run_start = 63 - __builtin_clzll(heads & (~0ull >> (63 - lane)));
+        // per-lane (P side) and wave-uniform (Q side) views of the plan data
+        const gptr<int> lbra = glb(pd.bra), lket = glb(pd.ket), lao = glb(pd.ao_loc);
+        const gptr<double> lbx = glb(pd.bas_x), lby = glb(pd.bas_y), lbz = glb(pd.bas_z);
+        const cptr<int> ubra = cst(pd.bra), uket = cst(pd.ket), uao = cst(pd.ao_loc);
+        const cptr<int> uperm_q = cst(tk.perm_q);
+        const cptr<double> ubx = cst(pd.bas_x), uby = cst(pd.bas_y), ubz = cst(pd.bas_z);
+        const int gp = tk.bas_p0 + ppos;
+        const int sa = pvalid ? lbra[gp] : -1;
+        const int sb = pvalid ? lket[gp] : 0;
+        const int pp0 = tk.prim_p0 + ppos * tk.nprim_p;
+        const int a0 = pvalid ? lao[sa] : 0;
+        const int b0 = pvalid ? lao[sb] : 0;
+        double ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+        double p_first[5] = {1, 0, 0, 0, 0};
+        if (pvalid) {
+            ra[0] = lbx[sa]; ra[1] = lby[sa]; ra[2] = lbz[sa];
+            rb[0] = lbx[sb]; rb[1] = lby[sb]; rb[2] = lbz[sb];
+            p_first[0] = glb(pd.a12)[pp0]; p_first[1] = glb(pd.e12)[pp0];
+            p_first[2] = glb(pd.x12)[pp0]; p_first[3] = glb(pd.y12)[pp0];
+            p_first[4] = glb(pd.z12)[pp0];
+        }
+        // runs of equal first shell a among the 64 lanes (of equal second
+        // shell b for the b-side role, whose lists are sorted by b)
+        const int rkey = ROLE == 2 ? (pvalid ? sb : -1) : sa;
+        const int key_next = __shfl_down(rkey, 1, 64);
+        const bool tail = lane == 63 || key_next != rkey;
+        const int key_prev = __shfl_up(rkey, 1, 64);
+        const unsigned long long heads = __ballot(lane == 0 || key_prev != rkey);
+        const int run_start = 63 - __builtin_clzll(heads & (~0ull >> (63 - lane)));
 
-    const int nao = jk.nao;
-    const size_t nao2 = (size_t)nao * nao;
-    for (int idm = 0; idm < jk.n_dm; ++idm) {
-        const gptr<double> D = glb(jk.dm + idm * nao2);      // per-lane reads
-        const cptr<double> DU = cst(jk.dm + idm * nao2);     // wave-uniform reads
-        gdouble *VJ = jk.vj ? glb_rw(jk.vj + idm * nao2) : nullptr;
-        gdouble *VK = jk.vk ? glb_rw(jk.vk + idm * nao2) : nullptr;
-        double dab[NA * NB];
+        const int nao = jk.nao;
+        const size_t nao2 = (size_t)nao * nao;
+        for (int idm = 0; idm < jk.n_dm; ++idm) {
+            const gptr<double> D = glb(jk.dm + idm * nao2);      // per-lane reads
+            const cptr<double> DU = cst(jk.dm + idm * nao2);     // wave-uniform reads
+            gdouble *VJ = jk.vj ? glb_rw(jk.vj + idm * nao2) : nullptr;
+            gdouble *VK = jk.vk ? glb_rw(jk.vk + idm * nao2) : nullptr;
+            double dab[NA * NB];
 #pragma unroll
-        for (int x = 0; x < NA; ++x)
+            for (int x = 0; x < NA; ++x)
 #pragma unroll
-            for (int y = 0; y < NB; ++y) dab[x * NB + y] = pvalid ? D[(size_t)(a0 + x) * nao + b0 + y] : 0.0;
-        double jab[NA * NB], kac[NA * NC], kbc[NB * NC];
+                for (int y = 0; y < NB; ++y) dab[x * NB + y] = pvalid ? D[(size_t)(a0 + x) * nao + b0 + y] : 0.0;
+            double jab[NA * NB], kac[NA * NC], kbc[NB * NC];
 #pragma unroll
-        for (int n = 0; n < NA * NB; ++n) jab[n] = 0.0;
+            for (int n = 0; n < NA * NB; ++n) jab[n] = 0.0;
 #pragma unroll
-        for (int n = 0; n < NA * NC; ++n) kac[n] = 0.0;
+            for (int n = 0; n < NA * NC; ++n) kac[n] = 0.0;
 #pragma unroll
-        for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
+            for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
 
-        int gq = tk.bas_q0 + uperm_q[q0];
-        int sc = __builtin_amdgcn_readfirstlane(ubra[gq]);
-        // D(b,c), D(a,c): loaded at the start of every run of equal c
-        double dbc[NB * NC], dac[NA * NC];
-        bool new_c = true;
-        for (int q = q0; q < q1; ++q) {
-            const int qpos = __builtin_amdgcn_readfirstlane(uperm_q[q]);
-            gq = tk.bas_q0 + qpos;
-            const int sd = __builtin_amdgcn_readfirstlane(uket[gq]);
-            const int c0 = uao[sc], d0 = uao[sd];
-            bool active = pvalid && q < qlim;
-            double norm = TWO_PI_52;
-            if (tk.diag) {
-                active = active && ppos >= qpos;
-                if (ppos == qpos) norm *= 0.5;
-            }
-            T g[C::NF];
-            if (active) {
-                const double rc[3] = {ubx[sc], uby[sc], ubz[sc]};
-                const double rd[3] = {ubx[sd], uby[sd], ubz[sd]};
-                if constexpr (RYS_LDS)
-                    eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
-                                                 tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
-                                                 norm, ra, rb, rc, rd, 0, rys_tab, p_first);
-                else
-                    eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
-                                                 tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
-                                                 norm, ra, rb, rc, rd, 0, nullptr, p_first);
-            } else {
+            int gq = tk.bas_q0 + uperm_q[q0];
+            int sc = __builtin_amdgcn_readfirstlane(ubra[gq]);
+            // D(b,c), D(a,c): loaded at the start of every run of equal c
+            double dbc[NB * NC], dac[NA * NC];
+            bool new_c = true;
+            for (int q = q0; q < q1; ++q) {
+                const int qpos = __builtin_amdgcn_readfirstlane(uperm_q[q]);
+                gq = tk.bas_q0 + qpos;
+                const int sd = __builtin_amdgcn_readfirstlane(uket[gq]);
+                const int c0 = uao[sc], d0 = uao[sd];
+                bool active = pvalid && q < qlim;
+                double norm = TWO_PI_52;
+                if (tk.diag) {
+                    active = active && ppos >= qpos;
+                    if (ppos == qpos) norm *= 0.5;
+                }
+                T g[C::NF];
+                if (active) {
+                    const double rc[3] = {ubx[sc], uby[sc], ubz[sc]};
+                    const double rd[3] = {ubx[sd], uby[sd], ubz[sd]};
+                    if constexpr (RYS_LDS)
+                        eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
+                                                     tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
+                                                     norm, ra, rb, rc, rd, 0, rys_tab, p_first);
+                    else
+                        eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
+                                                     tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
+                                                     norm, ra, rb, rc, rd, 0, nullptr, p_first);
+                } else {
 #pragma unroll
-                for (int n = 0; n < C::NF; ++n) g[n] = (T)0;
-            }
+                    for (int n = 0; n < C::NF; ++n) g[n] = (T)0;
+                }
 #define G(x, y, z, w) ((double)g[(((x) * NB + (y)) * NC + (z)) * ND + (w)])
-            if (VJ) {
-                // J(a,b) += g D(c,d)
+                if (ROLE != 2 && VJ) {
+                    // J(a,b) += g D(c,d)
 #pragma unroll
-                for (int z = 0; z < NC; ++z)
+                    for (int z = 0; z < NC; ++z)
 #pragma unroll
-                    for (int w = 0; w < ND; ++w) {
-                        const double dcd = DU[(size_t)(c0 + z) * nao + d0 + w];
+                        for (int w = 0; w < ND; ++w) {
+                            const double dcd = DU[(size_t)(c0 + z) * nao + d0 + w];
 #pragma unroll
-                        for (int x = 0; x < NA; ++x)
+                            for (int x = 0; x < NA; ++x)
 #pragma unroll
-                            for (int y = 0; y < NB; ++y) jab[x * NB + y] += G(x, y, z, w) * dcd;
-                    }
-                // J(c,d) += g D(a,b), whole wavefront
-#pragma unroll
-                for (int z = 0; z < NC; ++z)
-#pragma unroll
-                    for (int w = 0; w < ND; ++w) {
-                        double s = 0.0;
-#pragma unroll
-                        for (int x = 0; x < NA; ++x)
-#pragma unroll
-                            for (int y = 0; y < NB; ++y) s += G(x, y, z, w) * dab[x * NB + y];
-                        s = wave_sum63(s);
-                        if (lane == 63) atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, s);
-                    }
-            }
-            if (VK) {
-                // K(a,c) += g D(b,d); K(b,c) += g D(a,d); K(a,d) += g D(b,c); K(b,d) += g D(a,c)
-                double dbd[NB * ND], dad[NA * ND];
-#pragma unroll
-                for (int w = 0; w < ND; ++w) {
-#pragma unroll
-                    for (int y = 0; y < NB; ++y)
-                        dbd[y * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + b0 + y] : 0.0;
-#pragma unroll
-                    for (int x = 0; x < NA; ++x)
-                        dad[x * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + a0 + x] : 0.0;
-                }
-                if (new_c) {
-#pragma unroll
-                    for (int z = 0; z < NC; ++z) {
-#pragma unroll
-                        for (int y = 0; y < NB; ++y)
-                            dbc[y * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + b0 + y] : 0.0;
-#pragma unroll
-                        for (int x = 0; x < NA; ++x)
-                            dac[x * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + a0 + x] : 0.0;
-                    }
-                }
-#pragma unroll
-                for (int x = 0; x < NA; ++x)
-#pragma unroll
-                    for (int y = 0; y < NB; ++y)
-#pragma unroll
-                        for (int z = 0; z < NC; ++z)
-#pragma unroll
-                            for (int w = 0; w < ND; ++w) {
-                                const double v = G(x, y, z, w);
-                                kac[x * NC + z] += v * dbd[y * ND + w];
-                                kbc[y * NC + z] += v * dad[x * ND + w];
-                            }
-                // K(a,d): segmented over runs of equal a, added by the run tails
-                {
-                    constexpr int NV = NA * ND;
-                    double val[NV];
-                    int off[NV];
-#pragma unroll
-                    for (int w = 0; w < ND; ++w)
-#pragma unroll
-                        for (int x = 0; x < NA; ++x) {
-                            double s = 0.0;
-#pragma unroll
-                            for (int y = 0; y < NB; ++y)
-#pragma unroll
-                                for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dbc[y * NC + z];
-                            val[w * NA + x] = run_scan(s, lane, run_start);
-                            off[w * NA + x] = tail && pvalid ? (d0 + w) * nao + a0 + x : -1;
+                                for (int y = 0; y < NB; ++y) jab[x * NB + y] += G(x, y, z, w) * dcd;
                         }
-                    if constexpr (pack_nv(NV) > 0) {
-                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
-                    } else {
+                    // J(c,d) += g D(a,b), whole wavefront
 #pragma unroll
-                        for (int v = 0; v < NV; ++v)
-                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
-                    }
-                }
-                // K(b,d): per lane
-                {
-                    constexpr int NV = NB * ND;
-                    double val[NV];
-                    int off[NV];
+                    for (int z = 0; z < NC; ++z)
 #pragma unroll
-                    for (int w = 0; w < ND; ++w)
-#pragma unroll
-                        for (int y = 0; y < NB; ++y) {
+                        for (int w = 0; w < ND; ++w) {
                             double s = 0.0;
 #pragma unroll
                             for (int x = 0; x < NA; ++x)
 #pragma unroll
-                                for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dac[x * NC + z];
-                            val[w * NB + y] = s;
-                            off[w * NB + y] = active ? (d0 + w) * nao + b0 + y : -1;
+                                for (int y = 0; y < NB; ++y) s += G(x, y, z, w) * dab[x * NB + y];
+                            s = wave_sum63(s);
+                            if (lane == 63) atomic_add(VJ + (size_t)(c0 + z) * nao + d0 + w, s);
                         }
-                    if constexpr (pack_nv(NV) > 0) {
-                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
-                    } else {
+                }
+                if (VK) {
+                    // K(a,c) += g D(b,d); K(b,c) += g D(a,d); K(a,d) += g D(b,c); K(b,d) += g D(a,c)
+                    double dbd[NB * ND], dad[NA * ND];
 #pragma unroll
-                        for (int v = 0; v < NV; ++v)
-                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
+                    for (int w = 0; w < ND; ++w) {
+#pragma unroll
+                        for (int y = 0; y < NB; ++y)
+                            dbd[y * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + b0 + y] : 0.0;
+#pragma unroll
+                        for (int x = 0; x < NA; ++x)
+                            dad[x * ND + w] = pvalid ? D[(size_t)(d0 + w) * nao + a0 + x] : 0.0;
+                    }
+                    if (new_c) {
+#pragma unroll
+                        for (int z = 0; z < NC; ++z) {
+#pragma unroll
+                            for (int y = 0; y < NB; ++y)
+                                dbc[y * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + b0 + y] : 0.0;
+#pragma unroll
+                            for (int x = 0; x < NA; ++x)
+                                dac[x * NC + z] = pvalid ? D[(size_t)(c0 + z) * nao + a0 + x] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int x = 0; x < NA; ++x)
+#pragma unroll
+                        for (int y = 0; y < NB; ++y)
+#pragma unroll
+                            for (int z = 0; z < NC; ++z)
+#pragma unroll
+                                for (int w = 0; w < ND; ++w) {
+                                    const double v = G(x, y, z, w);
+                                    if (ROLE != 2) kac[x * NC + z] += v * dbd[y * ND + w];
+                                    if (ROLE != 1) kbc[y * NC + z] += v * dad[x * ND + w];
+                                }
+                    // K(a,d): segmented over runs of equal a, added by the run tails
+                    if constexpr (ROLE != 2) {
+                        constexpr int NV = NA * ND;
+                        double val[NV];
+                        int off[NV];
+#pragma unroll
+                        for (int w = 0; w < ND; ++w)
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) {
+                                double s = 0.0;
+#pragma unroll
+                                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                                    for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dbc[y * NC + z];
+                                val[w * NA + x] = run_scan(s, lane, run_start);
+                                off[w * NA + x] = tail && pvalid ? (d0 + w) * nao + a0 + x : -1;
+                            }
+                        if constexpr (pack_nv(NV) > 0) {
+                            packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v)
+                                if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
+                        }
+                    }
+                    // K(b,d): per lane (segmented over runs of equal b in the b-side role)
+                    if constexpr (ROLE != 1) {
+                        constexpr int NV = NB * ND;
+                        double val[NV];
+                        int off[NV];
+#pragma unroll
+                        for (int w = 0; w < ND; ++w)
+#pragma unroll
+                            for (int y = 0; y < NB; ++y) {
+                                double s = 0.0;
+#pragma unroll
+                                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                                    for (int z = 0; z < NC; ++z) s += G(x, y, z, w) * dac[x * NC + z];
+                                if constexpr (ROLE == 2) {
+                                    val[w * NB + y] = run_scan(s, lane, run_start);
+                                    off[w * NB + y] = tail && pvalid ? (d0 + w) * nao + b0 + y : -1;
+                                } else {
+                                    val[w * NB + y] = s;
+                                    off[w * NB + y] = active ? (d0 + w) * nao + b0 + y : -1;
+                                }
+                            }
+                        if constexpr (pack_nv(NV) > 0) {
+                            packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v)
+                                if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
+                        }
                     }
                 }
-            }
 #undef G
-            // end of a run of equal c: flush K(a,c), K(b,c)
-            int sc_next = -1;
-            if (q + 1 < q1) {
-                sc_next = __builtin_amdgcn_readfirstlane(
-                    ubra[tk.bas_q0 + uperm_q[q + 1]]);
-            }
-            if (VK && sc_next != sc) {
-                {
-                    constexpr int NV = NA * NC;
-                    double val[NV];
-                    int off[NV];
+                // end of a run of equal c: flush K(a,c), K(b,c)
+                int sc_next = -1;
+                if (q + 1 < q1) {
+                    sc_next = __builtin_amdgcn_readfirstlane(
+                        ubra[tk.bas_q0 + uperm_q[q + 1]]);
+                }
+                if (VK && sc_next != sc) {
+                    if constexpr (ROLE != 2) {
+                        constexpr int NV = NA * NC;
+                        double val[NV];
+                        int off[NV];
 #pragma unroll
-                    for (int z = 0; z < NC; ++z)
+                        for (int z = 0; z < NC; ++z)
 #pragma unroll
-                        for (int x = 0; x < NA; ++x) {
-                            val[z * NA + x] = run_scan(kac[x * NC + z], lane, run_start);
-                            off[z * NA + x] = tail && pvalid ? (c0 + z) * nao + a0 + x : -1;
-                            kac[x * NC + z] = 0.0;
+                            for (int x = 0; x < NA; ++x) {
+                                val[z * NA + x] = run_scan(kac[x * NC + z], lane, run_start);
+                                off[z * NA + x] = tail && pvalid ? (c0 + z) * nao + a0 + x : -1;
+                                kac[x * NC + z] = 0.0;
+                            }
+                        if constexpr (pack_nv(NV) > 0) {
+                            packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v)
+                                if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
                         }
-                    if constexpr (pack_nv(NV) > 0) {
-                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
-                    } else {
+                    }
+                    if constexpr (ROLE != 1) {
+                        constexpr int NV = NB * NC;
+                        double val[NV];
+                        int off[NV];
 #pragma unroll
-                        for (int v = 0; v < NV; ++v)
-                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
+                        for (int z = 0; z < NC; ++z)
+#pragma unroll
+                            for (int y = 0; y < NB; ++y) {
+                                if constexpr (ROLE == 2) {
+                                    val[z * NB + y] = run_scan(kbc[y * NC + z], lane, run_start);
+                                    off[z * NB + y] =
+                                        tail && pvalid ? (c0 + z) * nao + b0 + y : -1;
+                                } else {
+                                    val[z * NB + y] = kbc[y * NC + z];
+                                    off[z * NB + y] = pvalid && kbc[y * NC + z] != 0.0
+                                                          ? (c0 + z) * nao + b0 + y
+                                                          : -1;
+                                }
+                                kbc[y * NC + z] = 0.0;
+                            }
+                        if constexpr (pack_nv(NV) > 0) {
+                            packed_atomic_add<NV>(VK, val, off, sv, so, lane);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v)
+                                if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
+                        }
                     }
                 }
-                {
-                    constexpr int NV = NB * NC;
-                    double val[NV];
-                    int off[NV];
-#pragma unroll
-                    for (int z = 0; z < NC; ++z)
-#pragma unroll
-                        for (int y = 0; y < NB; ++y) {
-                            val[z * NB + y] = kbc[y * NC + z];
-                            off[z * NB + y] =
-                                pvalid && kbc[y * NC + z] != 0.0 ? (c0 + z) * nao + b0 + y : -1;
-                            kbc[y * NC + z] = 0.0;
-                        }
-                    if constexpr (pack_nv(NV) > 0) {
-                        packed_atomic_add<NV>(VK, val, off, sv, so, lane);
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v)
-                            if (off[v] >= 0) atomic_add(VK + off[v], val[v]);
-                    }
-                }
+                new_c = sc_next != sc;
+                sc = sc_next;
             }
-            new_c = sc_next != sc;
-            sc = sc_next;
-        }
-        if (VJ && pvalid) {
+            if (ROLE != 2 && VJ && pvalid) {
 #pragma unroll
-            for (int x = 0; x < NA; ++x)
+                for (int x = 0; x < NA; ++x)
 #pragma unroll
-                for (int y = 0; y < NB; ++y)
-                    if (jab[x * NB + y] != 0.0)
-                        atomic_add(VJ + (size_t)(a0 + x) * nao + b0 + y, jab[x * NB + y]);
+                    for (int y = 0; y < NB; ++y)
+                        if (jab[x * NB + y] != 0.0)
+                            atomic_add(VJ + (size_t)(a0 + x) * nao + b0 + y, jab[x * NB + y]);
+            }
         }
+    };
+    if (!ticket) {
+        const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+        if (item < nitems) run_item(item);
+        return;
+    }
+    // persistent: every wavefront takes work items from the launch's
+    // counter until the list is exhausted
+    for (;;) {
+        int tk0 = 0;
+        if ((threadIdx.x & 63) == 0)
+            tk0 = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int item = __builtin_amdgcn_readfirstlane(tk0);
+        if (item >= nitems) break;
+        run_item(item);
     }
 }
 
-typedef void (*RowKernel)(const TaskDesc *, const int4 *, int);
-typedef RowKernel RowKernelTable[2][BQC_NPAIRID][BQC_NPAIRID];
+typedef void (*RowKernel)(const TaskDesc *, const int4 *, int, int *);
+// [role][precision][bra pair id][ket pair id]: role 0 the single pass, roles
+// 1 and 2 the a-side and b-side passes of split classes (FP64 only)
+typedef RowKernel RowKernelTable[3][2][BQC_NPAIRID][BQC_NPAIRID];
 
 // Classes digested with the wavefront-stationary kernel, per precision.
 // Up to 108 Cartesian components the FP64 block and the lane accumulators
@@ -455,11 +500,23 @@ constexpr bool row_fp32() {
     return c == 2010 || c == 2100 || c == 2120 || c == 2210 || c == 3020 || c == 3110;
 }
 
+// Row classes compiled with the a-side / b-side pair as well: the small s/p
+// classes, whose time goes to atomic requests rather than arithmetic.  Whether
+// a build uses the pair is chosen per class at run time (BQCset_row_split).
 template <int LA, int LB, int LC, int LD>
-static void reg_row(RowKernelTable &tab) {
-    if constexpr (use_rows<LA, LB, LC, LD, double>())
-        tab[0][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, double>;
-    if constexpr (use_rows<LA, LB, LC, LD, float>() && row_fp32<LA, LB, LC, LD>())
-        tab[1][LA * (LA + 1) / 2 + LB][LC * (LC + 1) / 2 + LD] = jk_rows<LA, LB, LC, LD, float>;
+constexpr bool row_split_candidate() {
+    return Cls<LA, LB, LC, LD>::NF <= 27 && LA <= 2 && LC <= 1;
 }
 
+template <int LA, int LB, int LC, int LD>
+static void reg_row(RowKernelTable &tab) {
+    constexpr int pab = LA * (LA + 1) / 2 + LB, pcd = LC * (LC + 1) / 2 + LD;
+    if constexpr (use_rows<LA, LB, LC, LD, double>())
+        tab[0][0][pab][pcd] = jk_rows<LA, LB, LC, LD, double>;
+    if constexpr (use_rows<LA, LB, LC, LD, float>() && row_fp32<LA, LB, LC, LD>())
+        tab[0][1][pab][pcd] = jk_rows<LA, LB, LC, LD, float>;
+    if constexpr (use_rows<LA, LB, LC, LD, double>() && row_split_candidate<LA, LB, LC, LD>()) {
+        tab[1][0][pab][pcd] = jk_rows<LA, LB, LC, LD, double, 1>;
+        tab[2][0][pab][pcd] = jk_rows<LA, LB, LC, LD, double, 2>;
+    }
+}

@@ -128,17 +128,27 @@ static QKernel q_kernel(int li, int lj) {
     return nullptr;
 }
 
+// Pairs with a g or h shell (jk_general.hip).
+int bqc_q_cond_general(double *q, const int *atm, const int *bas, const double *env, int nbas,
+                       int p0, int p1, int q0, int q1, int diag, hipStream_t stream);
+
 extern "C" int BQCq_cond(double *q, const int *atm, const int *bas, const double *env,
                          int nbas, const int *grp_offsets, int ngroups, const int *grp_l,
                          hipStream_t stream) {
     for (int g = 0; g < ngroups; ++g) {
         for (int h = 0; h <= g; ++h) {
-            QKernel k = q_kernel(grp_l[g], grp_l[h]);
-            if (!k) return 1;
             int p0 = grp_offsets[g], p1 = grp_offsets[g + 1];
             int q0 = grp_offsets[h], q1 = grp_offsets[h + 1];
             long long n = (long long)(p1 - p0) * (q1 - q0);
             if (n == 0) continue;
+            if (grp_l[g] > 3 || grp_l[h] > 3) {
+                if (grp_l[g] > 5 || grp_l[h] > 5 ||
+                    bqc_q_cond_general(q, atm, bas, env, nbas, p0, p1, q0, q1, g == h, stream))
+                    return 1;
+                continue;
+            }
+            QKernel k = q_kernel(grp_l[g], grp_l[h]);
+            if (!k) return 1;
             hipLaunchKernelGGL(k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, q,
                                atm, bas, env, nbas, p0, p1, q0, q1, g == h);
             if (hipGetLastError() != hipSuccess) return 1;

@@ -230,10 +230,12 @@ class Mole:
         templates = {}
         for ia, (sym, _) in enumerate(atoms):
             bname = self.basis[sym] if isinstance(self.basis, dict) else self.basis
-            key = (bname, sym)
+            key = (bname if isinstance(bname, str) else repr(bname), sym)
             if key not in templates:
                 shells = []
-                for shell in load_basis(bname, sym):
+                # a basis name, or PySCF's explicit form [[l, [e, c1, ...], ...], ...]
+                src = load_basis(bname, sym) if isinstance(bname, str) else bname
+                for shell in src:
                     l = shell[0]
                     ec = numpy.array(shell[1:])
                     es, cs = ec[:, 0], ec[:, 1:]

@@ -41,6 +41,8 @@ _SIGNATURES = {
     'BQCgraph_destroy': (None, [c_void_p]),
     'BQCset_tile': (c_int, [c_int, c_int, c_int, c_int]),
     'BQCget_tile': (c_int, [c_int, c_int, c_int, c_int]),
+    'BQCset_launch_mode': (c_int, [c_int]),
+    'BQCset_row_split': (c_int, [c_int, c_int]),
     'BQCplan_tasks': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double,
                               c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     'BQCdm_cond': (c_int, [c_void_p] * 7 + [c_int, c_int, c_int, c_void_p]),

@@ -19,7 +19,7 @@ fallback.
 Differences from the reference, all deliberate:
 * ``with_j=False`` / ``with_k=False`` work (the reference raises NameError,
   hf.py:168-169);
-* shells with l > 3 raise NotImplementedError (the reference hands l > 5 to
+* shells with l > 5 raise NotImplementedError (the reference hands them to
   PySCF on the CPU, hf.py:185-218, which is not part of this package);
 * ``gpus`` is accepted and ignored: the build runs on the current device.
 """
@@ -36,7 +36,7 @@ from byteqc_amd.gto.mole import (ANG_OF, NPRIM_OF, NCTR_OF, PTR_EXP, PTR_COEFF, 
                                  gto_norm)
 from byteqc_amd.lib import load_library
 
-LMAX_ON_GPU = 3
+LMAX_ON_GPU = 5
 ANGULAR = 'spdfghik'
 
 

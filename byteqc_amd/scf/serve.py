@@ -121,7 +121,7 @@ class Server:
 
     def __init__(self, hbm_budget=None, max_active=64, precision='mixed_then_fp64',
                  conv_tol=1e-8, switch_de=1e-5, max_cycle=80, fp32_cut=1e-7, log=None,
-                 lanes=(800, 1600)):
+                 lanes=(800, 1600), init_guess='atom'):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * free)
@@ -131,7 +131,7 @@ class Server:
         self._setup_stream = None
         self.max_active = max_active
         self.scf_defaults = dict(precision=precision, conv_tol=conv_tol, switch_de=switch_de,
-                                 max_cycle=max_cycle, fp32_cut=fp32_cut)
+                                 max_cycle=max_cycle, fp32_cut=fp32_cut, init_guess=init_guess)
         self.fp32_cut = fp32_cut
         self.queue = collections.deque()
         self.active = []

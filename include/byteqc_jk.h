@@ -178,6 +178,28 @@ void BQCgraph_destroy(void *graph);
  * nr_jk_driver.cu:46-225).  Returns 0 ok, 1 bad argument. */
 int BQCset_tile(int kind, int cls, int prec, int ck);
 
+/* Whether the row-kernel class cls (code li*1000+lj*100+lk*10+ll as
+ * evaluated; -1: all classes, clearing per-class settings) runs as two
+ * passes in FP64 builds: an a-side pass (J(a,b), J(c,d), K(a,c), K(a,d) over
+ * pair lists sorted by first shell) and a b-side pass (K(b,c), K(b,d) over the
+ * same pairs sorted by second shell, summed over runs of equal b before the
+ * atomics), twice the integral work for several times fewer atomic
+ * requests.  Only the small s/p classes have the two passes compiled;
+ * default on.  Process-wide; not to be called while another thread plans a
+ * build.  No counterpart in the reference.  Returns 0 ok, 1 bad argument or
+ * too many per-class settings. */
+int BQCset_row_split(int cls, int on);
+
+/* Launch shape of the row and wavefront kernels: 1 (default) persistent
+ * grids -- as many workgroups as stay resident on the device at once, each
+ * wavefront pulling work items from a counter in the build's work list until
+ * it is exhausted; 0 one workgroup per work item (four items per row-kernel
+ * workgroup).  Same quartets and sums either way.  Process-wide; not to be
+ * called while another thread plans a build.  No counterpart in the
+ * reference (it launches one grid per task and bin, nr_jk_driver.cu:252-280).
+ * Returns 0 ok, 1 bad argument. */
+int BQCset_launch_mode(int persistent);
+
 /* The tile a build of a molecule with nao Cartesian AOs planned now would
  * use for (kind, cls, prec); -1 for a bad kind.  No counterpart in the
  * reference. */

@@ -31,10 +31,11 @@
 #include <stdio.h>
 #include <omp.h>
 
-#define ORC_NMAX 9
+#define ORC_NMAX 11
 #define ORC_DEG 20
 #define ORC_NQUAD 160
-#define ORC_LMAX 4
+#define ORC_LMAX 5
+#define ORC_BLK (NCART(ORC_LMAX) * NCART(ORC_LMAX) * NCART(ORC_LMAX) * NCART(ORC_LMAX))
 
 /* slots of _atm/_bas (cuobc/lib/gint/gint.h:28-46) */
 #define ATOM_OF 0
@@ -522,7 +523,7 @@ void oracle_q_cond(double *qc, const int *atm, int natm, const int *bas, int nba
     (void)natm;
 #pragma omp parallel
     {
-        double *buf = (double *)malloc(sizeof(double) * 50625);
+        double *buf = (double *)malloc(sizeof(double) * ORC_BLK);
 #pragma omp for schedule(dynamic, 4)
         for (int ish = 0; ish < nbas; ish++) {
             for (int jsh = 0; jsh <= ish; jsh++) {
@@ -605,7 +606,7 @@ int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
         int tid = omp_get_thread_num();
         double *vjp = vpriv + (size_t)tid * 2 * n_dm * nao2;
         double *vkp = vjp + (size_t)n_dm * nao2;
-        double *blk = (double *)malloc(sizeof(double) * 50625);
+        double *blk = (double *)malloc(sizeof(double) * ORC_BLK);
 #pragma omp for schedule(dynamic, 16)
         for (long long item = 0; item < c; item++) {
             int t = it_task[item], kl = it_kl[item], kl_bin = it_bin[item];
@@ -682,3 +683,10 @@ int oracle_jk(int ntask, const int *task_cp, const int *task_bins, int nbins,
     if (nquartets_out) *nquartets_out = nq;
     return 0;
 }
+
+/* Threads of the oracle's OpenMP regions (for single-thread baselines). */
+void oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int oracle_get_threads(void) { return omp_get_max_threads(); }

@@ -12,7 +12,7 @@ Everything here is computed from the mathematics with mpmath at 80 digits:
   give the weights.
 
 Output: rys_golden.npz with arrays x (nx,), boys (nx, 20) and, for every
-n in 1..9, roots_n / weights_n (nx, n) as float64 (nodes as u = s/(1-s), the
+n in 1..11, roots_n / weights_n (nx, n) as float64 (nodes as u = s/(1-s), the
 form the integral recurrences consume, and as s).
 
 Run:  python tests/golden/make_rys_golden.py
@@ -23,7 +23,7 @@ import mpmath
 import numpy
 
 mpmath.mp.dps = 80
-NMAX = 9
+NMAX = 11
 MMAX = 20
 
 

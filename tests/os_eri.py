@@ -23,9 +23,10 @@ def boys(m, t):
     return mpmath.gammainc(a, 0, t) / (2 * t ** a)
 
 
-def primitive_eri(la, lb, lc, ld, alpha, beta, gamma, delta, A, B, C, D):
-    """dict[(a, b, c, d)] -> integral for all Cartesian components (tuples of
-    three exponents)."""
+def primitive_eri(la, lb, lc, ld, alpha, beta, gamma, delta, A, B, C, D, subset=None):
+    """Nested lists [a][b][c][d] of the integrals over all Cartesian
+    components (storage order), or, with ``subset`` a list of component index
+    quartets (x, y, z, w), the list of those integrals only."""
     mp = mpmath.mpf
     alpha, beta, gamma, delta = map(mp, (alpha, beta, gamma, delta))
     A, B, C, D = ([mp(x) for x in v] for v in (A, B, C, D))
@@ -87,5 +88,8 @@ def primitive_eri(la, lb, lc, ld, alpha, beta, gamma, delta, A, B, C, D):
     def comps(l):
         return [(lx, ly, l - lx - ly) for lx in range(l, -1, -1) for ly in range(l - lx, -1, -1)]
 
+    if subset is not None:
+        ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+        return [hrr(ca[x], cb[y], cc[z], cd[w]) for x, y, z, w in subset]
     return [[[[hrr(a, b, c, d) for d in comps(ld)] for c in comps(lc)]
              for b in comps(lb)] for a in comps(la)]

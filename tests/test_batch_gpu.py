@@ -127,3 +127,54 @@ def test_autotuned_tiles_keep_results():
         assert lib.BQCget_tile(1, 1000, 0, 3000) == 16
     finally:
         autotune.reset()
+
+
+def test_persistent_and_per_item_launches_agree_on_c3_subbatch():
+    """Molecules drawn like the 256-molecule batch (4-40 waters, def2-SVP,
+    density 1 g/cm^3): the persistent work-queue launches (default), the
+    one-workgroup-per-item launches and a replayed graph of the persistent
+    launches give the same J/K, equal to single builds and, for the smallest
+    molecule, to the oracle."""
+    torch = _torch_gpu()
+    from byteqc_amd.lib import load_library
+    from byteqc_amd.scf import hf
+    from byteqc_amd.scf.batch import BatchJK
+    from byteqc_amd.systems import batch_specs, water_mole, random_density
+    from oracle import oracle as orc
+    specs = batch_specs(256)[:6]
+    mols = [water_mole(s['nwater'], s['box'], s['seed'], s['basis']) for s in specs]
+    dms = [random_density(m.nao, 5 * s['nwater'], s['dm_seed']) for m, s in zip(mols, specs)]
+    lib = load_library()
+    bj = BatchJK(mols)
+    try:
+        assert lib.BQCset_launch_mode(1) == 0
+        pers = bj.build(dms)
+        assert lib.BQCset_launch_mode(0) == 0
+        stat = bj.build(dms)
+    finally:
+        lib.BQCset_launch_mode(1)
+    assert lib.BQCset_launch_mode(2) == 1
+    bj.capture(dms)
+    for rep in range(3):
+        x = torch.randn(200, 200, dtype=torch.float64, device='cuda')
+        (x @ x).sum().item()
+        torch.cuda.synchronize()
+        rep_out = bj.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(pers, rep_out):
+            assert _maxdiff(a[0].cpu(), b[0].cpu()) < 1e-11
+            assert _maxdiff(a[1].cpu(), b[1].cpu()) < 1e-11
+    bj.release()
+    for i, (m, d) in enumerate(zip(mols, dms)):
+        dj = _maxdiff(pers[i][0].cpu(), stat[i][0].cpu())
+        dk = _maxdiff(pers[i][1].cpu(), stat[i][1].cpu())
+        j1, k1 = hf.get_jk(m, d)
+        print(f'{specs[i]["nwater"]} H2O: persistent vs per-item max|dJ| {dj:.2e} '
+              f'max|dK| {dk:.2e}', flush=True)
+        assert dj < 1e-11 and dk < 1e-11
+        assert _maxdiff(pers[i][0].cpu(), j1) < 1e-11
+        assert _maxdiff(pers[i][1].cpu(), k1) < 1e-11
+    small = min(range(len(mols)), key=lambda i: mols[i].nao)
+    jr, kr = orc.get_jk(mols[small], dms[small])
+    assert _maxdiff(pers[small][0].cpu(), jr) < 1e-10
+    assert _maxdiff(pers[small][1].cpu(), kr) < 1e-10

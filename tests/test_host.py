@@ -114,8 +114,10 @@ def test_split_l_ctr_groups():
 
 
 def test_rys_table_matches_mpmath():
+    """Both compiled tables (n = 1..7 and 8..11 roots) against the 80-digit
+    mpmath vectors."""
     gold = numpy.load(os.path.join(ROOT, 'tests', 'golden', 'rys_golden.npz'))
-    for n in range(1, 8):
+    for n in range(1, 12):
         for ix, x in enumerate(gold['x']):
             u, w = rys_table.rys(n, x)
             assert numpy.allclose(u, gold[f'u_{n}'][ix], rtol=2e-14, atol=0), (n, x)

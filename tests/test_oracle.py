@@ -34,7 +34,7 @@ def test_boys_matches_mpmath(gold):
         assert numpy.allclose(f, gold['boys'][ix], rtol=1e-14, atol=0), x
 
 
-@pytest.mark.parametrize('n', range(1, 10))
+@pytest.mark.parametrize("n", range(1, 12))
 def test_rys_matches_mpmath(gold, n):
     for ix, x in enumerate(gold['x']):
         u, w = orc.rys(n, x)
@@ -43,7 +43,7 @@ def test_rys_matches_mpmath(gold, n):
         assert numpy.allclose(w, gold[f'w_{n}'][ix], rtol=1e-14, atol=0), (n, x)
 
 
-@pytest.mark.parametrize('n', [1, 3, 5, 7])
+@pytest.mark.parametrize('n', [1, 3, 5, 7, 9, 11])
 def test_rys_reproduces_boys_moments(n):
     rng = numpy.random.default_rng(n)
     for x in numpy.concatenate([rng.uniform(0, 100, 30), [0., 1e-9, 47.5, 500.]]):
@@ -54,7 +54,7 @@ def test_rys_reproduces_boys_moments(n):
             assert abs(numpy.sum(w * s ** k) - f[k]) <= 1e-13 * f[k], (n, x, k)
 
 
-@pytest.mark.parametrize('n', range(1, 8))
+@pytest.mark.parametrize('n', range(1, 12))
 def test_rys_small_x_is_gauss_legendre(n):
     t, wt = numpy.polynomial.legendre.leggauss(2 * n)
     u, w = orc.rys(n, 0.)
@@ -99,6 +99,44 @@ def test_eri_matches_obara_saika(ls):
     # block-relative: HRR cancellation leaves ~1e-13 of the block maximum on
     # elements that are orders of magnitude smaller than it
     assert numpy.abs(got - ref).max() <= 2e-13 * numpy.abs(ref).max()
+
+
+@pytest.mark.parametrize('ls', [(4, 0, 0, 0), (4, 1, 2, 0), (5, 0, 1, 0), (4, 4, 0, 0),
+                                (5, 2, 1, 1), (3, 4, 0, 5)])
+def test_eri_matches_obara_saika_g_h_shells(ls):
+    """g and h shells (8-11 Rys roots in the larger classes), whole blocks."""
+    rng = numpy.random.default_rng(97 + sum(l * 7 ** i for i, l in enumerate(ls)))
+    shells = [(l, float(rng.uniform(0.3, 2.0)), rng.uniform(-1.2, 1.2, 3)) for l in ls]
+    atm, bas, env = _prim_shell_mol(shells)
+    got = orc.eri_shells(0, 1, 2, 3, atm, bas, env)
+    ref = os_eri.primitive_eri(*ls, *[s[1] for s in shells], *[s[2] for s in shells])
+    scale = 1.
+    for l, a, _ in shells:
+        scale *= _fac_sp(l) * float(gto_norm(l, a))
+    ref = numpy.array(ref, dtype=float) * scale
+    assert got.shape == ref.shape
+    assert numpy.abs(got - ref).max() <= 1e-12 * numpy.abs(ref).max()
+
+
+@pytest.mark.parametrize('ls', [(4, 4, 4, 4), (5, 5, 5, 5), (5, 4, 5, 3)])
+def test_eri_matches_obara_saika_largest_classes(ls):
+    """(gg|gg), (hh|hh), (hg|hf): 9-11 roots; a random sample of components
+    (the mpmath recursion over every component would take minutes)."""
+    rng = numpy.random.default_rng(11 + sum(ls))
+    shells = [(l, float(rng.uniform(0.4, 1.8)), rng.uniform(-1.0, 1.0, 3)) for l in ls]
+    atm, bas, env = _prim_shell_mol(shells)
+    got = orc.eri_shells(0, 1, 2, 3, atm, bas, env)
+    nf = [(l + 1) * (l + 2) // 2 for l in ls]
+    subset = [tuple(int(rng.integers(n)) for n in nf) for _ in range(12)]
+    ref = os_eri.primitive_eri(*ls, *[s[1] for s in shells], *[s[2] for s in shells],
+                               subset=subset)
+    scale = 1.
+    for l, a, _ in shells:
+        scale *= _fac_sp(l) * float(gto_norm(l, a))
+    ref = numpy.array([float(v) for v in ref]) * scale
+    vals = numpy.array([got[q] for q in subset])
+    big = numpy.abs(got).max()
+    assert numpy.abs(vals - ref).max() <= 1e-12 * big
 
 
 def test_ssss_closed_form():
