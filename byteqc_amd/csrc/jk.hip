@@ -892,6 +892,71 @@ static int plan_molecules(BuildPlanT<V> &plan, const MolArgs *m, int nmol, int n
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free build (BQCbuild_jk_noalloc).  The device work list lives in
+// the file-scope array bqc_noalloc_scratch (each GPU the library runs on
+// holds its own copy until the process ends), the host side of the plan in a
+// static arena; neither the device nor the host heap is touched.  The
+// array's host-side name is not a device address: the device address of
+// each GPU's copy comes from hipGetSymbolAddress, once per device, and only
+// that address is handed to copies and kernels.
+// ---------------------------------------------------------------------------
+#define BQC_NOALLOC_BYTES (64ull << 20)        // device work list, per GPU
+#define BQC_NOALLOC_ARENA (256ull << 20)       // host plan arena (growth slack)
+#define BQC_NOALLOC_DEVICES 64
+static __device__ __align__(256) char bqc_noalloc_scratch[BQC_NOALLOC_BYTES];
+
+struct HostArena {
+    alignas(256) char mem[BQC_NOALLOC_ARENA];
+    size_t off = 0;
+};
+static HostArena g_arena;  // guarded by g_noalloc_mu
+// Host copy of the work list handed to the device copy (pageable); refilled
+// only after the previous copy out of it has completed.
+alignas(256) static char g_noalloc_host[BQC_NOALLOC_BYTES];
+static std::mutex g_noalloc_mu;
+static char *g_noalloc_dev[BQC_NOALLOC_DEVICES];
+static hipEvent_t g_noalloc_copied = nullptr;
+static bool g_noalloc_pending = false;
+
+struct ArenaFull {};
+template <class T>
+struct ArenaAlloc {
+    using value_type = T;
+    ArenaAlloc() = default;
+    template <class U>
+    ArenaAlloc(const ArenaAlloc<U> &) {}
+    T *allocate(size_t n) {
+        const size_t off = (g_arena.off + 15) & ~(size_t)15;
+        if (n > (BQC_NOALLOC_ARENA - off) / sizeof(T)) throw ArenaFull();
+        g_arena.off = off + n * sizeof(T);
+        return (T *)(g_arena.mem + off);
+    }
+    void deallocate(T *, size_t) {}
+    template <class U>
+    bool operator==(const ArenaAlloc<U> &) const { return true; }
+    template <class U>
+    bool operator!=(const ArenaAlloc<U> &) const { return false; }
+};
+template <class T> using ArenaVec = std::vector<T, ArenaAlloc<T>>;
+
+// Device address of this GPU's copy of the scratch array (set-up on the
+// first call on each device, with that device current).
+static char *noalloc_device_ptr() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= BQC_NOALLOC_DEVICES)
+        return nullptr;
+    if (!g_noalloc_dev[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(bqc_noalloc_scratch)) != hipSuccess || !ptr) {
+            fprintf(stderr, "bqcjk: hipGetSymbolAddress of the no-alloc scratch failed\n");
+            return nullptr;
+        }
+        g_noalloc_dev[dev] = (char *)ptr;
+    }
+    return g_noalloc_dev[dev];
+}
+
 
 extern "C" {
 
@@ -950,6 +1015,59 @@ int BQCbuild_jk_batch(int nmol, BasisProdCache **bps, double **vj, double **vk, 
     if (!err) err = stage_and_launch(plan, stream);
     g_last_flops[0] = plan.flops[0];
     g_last_flops[1] = plan.flops[1];
+    return err;
+}
+
+int BQCbuild_jk_noalloc(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
+                        int n_dm, int ntask, const int *task_cp, const int *task_bins,
+                        int nbins, const int *bra_order, int precision, const double *log_q,
+                        const double *task_dmax, double fp32_cut, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    char *dev = noalloc_device_ptr();
+    if (!dev) return 1;
+    if (!g_noalloc_copied &&
+        hipEventCreateWithFlags(&g_noalloc_copied, hipEventDisableTiming) != hipSuccess)
+        return 1;
+    MolArgs m = {bpcache, vj, vk, dm, nao, n_dm, ntask, task_cp, task_bins, bra_order,
+                 log_q, task_dmax};
+    g_arena.off = 0;
+    t_noalloc = true;
+    int err = 0;
+    try {
+        BuildPlanT<ArenaVec> plan;
+        err = plan_molecules(plan, &m, 1, nbins, precision, fp32_cut, stream);
+        g_last_flops[0] = plan.flops[0];
+        g_last_flops[1] = plan.flops[1];
+        if (!err && !plan.groups.empty()) {
+            StagedT<ArenaVec> st = stage_layout(plan);
+            if (st.bytes > BQC_NOALLOC_BYTES) {
+                fprintf(stderr,
+                        "bqcjk: work list of %zu bytes exceeds the %llu-byte scratch of "
+                        "BQCbuild_jk_noalloc\n",
+                        st.bytes, (unsigned long long)BQC_NOALLOC_BYTES);
+                err = 5;
+            } else {
+                if (g_noalloc_pending) {
+                    hipEventSynchronize(g_noalloc_copied);
+                    g_noalloc_pending = false;
+                }
+                stage_fill(plan, st, dev, g_noalloc_host);
+                if (hipMemcpyAsync(dev, g_noalloc_host, st.bytes, hipMemcpyHostToDevice,
+                                   stream) != hipSuccess)
+                    err = 1;
+                if (!err) {
+                    hipEventRecord(g_noalloc_copied, stream);
+                    g_noalloc_pending = true;
+                    err = launch_groups(plan, st, dev, stream);
+                }
+            }
+        }
+    } catch (const ArenaFull &) {
+        fprintf(stderr, "bqcjk: host plan exceeds the %llu-byte arena of BQCbuild_jk_noalloc\n",
+                (unsigned long long)BQC_NOALLOC_ARENA);
+        err = 5;
+    }
+    t_noalloc = false;
     return err;
 }
 

@@ -25,6 +25,9 @@ _SIGNATURES = {
     'BQCbuild_jk': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                             c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
                             c_double, c_void_p]),
+    'BQCbuild_jk_noalloc': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                    c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
+                                    c_void_p, c_void_p, c_double, c_void_p]),
     'BQCq_cond': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int,
                           c_void_p, c_void_p]),
     'BQCint1e': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,

@@ -140,6 +140,22 @@ int BQCbuild_jk(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int
                 const int *bra_order, int precision, const double *log_q,
                 const double *task_dmax, double fp32_cut, hipStream_t stream);
 
+/* BQCbuild_jk without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, work and return
+ * codes as BQCbuild_jk, plus 5 when the build's device work list exceeds the
+ * 64 MiB scratch array this library keeps per GPU (or its host plan the
+ * 256 MiB host arena).  The work list goes to a file-scope device array
+ * (one per GPU, held until the process ends) instead of memory from
+ * hipMalloc/hipHostMalloc; no device or host heap allocation is made.
+ * Every call on a GPU shares that GPU's array: calls on one stream are
+ * ordered, calls that overlap on one GPU (other streams, other threads)
+ * race on it.  Calls are serialised on the host. */
+int BQCbuild_jk_noalloc(BasisProdCache *bpcache, double *vj, double *vk, double *dm, int nao,
+                        int n_dm, int ntask, const int *task_cp, const int *task_bins,
+                        int nbins, const int *bra_order, int precision, const double *log_q,
+                        const double *task_dmax, double fp32_cut, hipStream_t stream);
+
 /* The same build for many independent molecules in one set of launches
  * (one launch per integral class and kernel kind across all molecules).
  * Arrays of length nmol hold each molecule's plan, device J/K/density
