@@ -178,10 +178,16 @@ def roofline(recs, steps, workload):
     peak = FP32_PEAK_TFLOPS if kind & 2 else FP64_PEAK_TFLOPS
     name = kernel_name(cls, kind, dom['split'])
     kern_ms = sum(v['ms'] for v in by_cls.values()) / steps
+    top = sorted(by_cls.items(), key=lambda kv: -kv[1]['ms'])[:12]
+    classes = {kernel_name(c, k, v['split']): {
+        'ms_per_build': v['ms'] / steps,
+        'tflops': v['flops'] / (v['ms'] * 1e-3) / 1e12 if v['ms'] > 0 else 0.0}
+        for (c, k), v in top}
     return {'bound': 'valu-fp32' if kind & 2 else 'valu-fp64', 'kernel': name,
             'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
             'traffic': measured_traffic(workload, name), 'launches': dom['n'],
-            'avg_launch_ms': avg_ms, 'flops_per_launch': flops_per_launch}, kern_ms
+            'avg_launch_ms': avg_ms, 'flops_per_launch': flops_per_launch,
+            'top_classes': classes}, kern_ms
 
 
 def log(msg):

@@ -225,9 +225,13 @@ static bool prof_on() { return g_prof && !t_noalloc; }
 // per item (four for the row kernels).  Set by BQCset_launch_mode.
 static bool g_persistent = true;
 
-// Workgroups of kern (nthr lanes) resident on the whole device at once:
-// occupancy per CU times the CU count.  Cached per kernel and device in a
-// fixed table (no heap, so the allocation-free entry may use it).
+// Workgroups of kern (nthr lanes) that fit on the whole device at once, from
+// the kernel's registers and LDS: waves per SIMD = 512 / VGPRs (granule 8, at
+// most 8), four SIMDs per CU, LDS per CU as the device reports it (at least
+// its per-workgroup limit), at most 32 workgroups per CU; never fewer than
+// the runtime's occupancy answer.  A persistent grid of this size keeps the
+// device full; extra workgroups would only wait.  Cached per kernel and
+// device in a fixed table (no heap, so the allocation-free entry may use it).
 #define BQC_OCC_CACHE 1024
 static unsigned resident_blocks(const void *kern, int nthr) {
     static std::mutex mu;
@@ -240,12 +244,34 @@ static unsigned resident_blocks(const void *kern, int nthr) {
     std::lock_guard<std::mutex> lock(mu);
     for (int i = 0; i < nkeys; ++i)
         if (keys[i] == kern && devs[i] == dev) return vals[i];
-    int per_cu = 0, ncu = 0;
     unsigned v = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nthr, 0) == hipSuccess &&
+    hipFuncAttributes a;
+    int ncu = 0, lds_cu = 0, lds_blk = 0, api = 0;
+    if (hipFuncGetAttributes(&a, kern) == hipSuccess &&
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        per_cu > 0 && ncu > 0)
+        ncu > 0) {
+        if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  dev) != hipSuccess)
+            lds_cu = 0;
+        if (hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess)
+            lds_blk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, kern, nthr, 0) != hipSuccess)
+            api = 0;
+        lds_cu = lds_cu > lds_blk ? lds_cu : lds_blk;
+        const int wpb = (nthr + 63) / 64;
+        const int vgpr = a.numRegs > 0 ? (a.numRegs + 7) / 8 * 8 : 512;
+        const int waves_simd = 512 / vgpr < 8 ? 512 / vgpr : 8;
+        int per_cu = 4 * waves_simd / wpb;
+        if (a.sharedSizeBytes > 0 && lds_cu > 0) {
+            const int by_lds = lds_cu / (int)a.sharedSizeBytes;
+            if (by_lds < per_cu) per_cu = by_lds;
+        }
+        if (per_cu > 32) per_cu = 32;
+        if (per_cu < api) per_cu = api;
+        if (per_cu < 1) per_cu = 1;
         v = (unsigned)per_cu * (unsigned)ncu;
+    }
     if (nkeys < BQC_OCC_CACHE) {
         keys[nkeys] = kern;
         devs[nkeys] = dev;
@@ -551,8 +577,8 @@ static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
     // split row class: a-side and b-side passes (FP64 builds of the class only)
     RowKernel ks[2] = {nullptr, nullptr};
     const int *order2 = nullptr;
-    if (!wv[0] && !t_noalloc && log_cut <= -1e299 && split_on(eval_cls(c)) &&
-        c.bp->bas_pair2shls) {
+    if (!wv[0] && !t_noalloc && log_cut <= -1e299 && jk.nao >= BQC_TILE_BIG_NAO &&
+        split_on(eval_cls(c)) && c.bp->bas_pair2shls) {
         for (int role = 1; role <= 2; ++role)
             ks[role - 1] = c.swap ? g_rows[role][0][c.pkl][c.pij] : g_rows[role][0][c.pij][c.pkl];
         if (ks[0] && ks[1] && scratch_ok((const void *)ks[0]) && scratch_ok((const void *)ks[1])) {

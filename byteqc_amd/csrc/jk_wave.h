@@ -399,20 +399,19 @@ __global__ void __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>()))
             wave_sync();
         }
     };
-    if (!ticket) {
-        const int item = (int)blockIdx.x;
-        if (item < nitems) run_item(item);
-        return;
-    }
-    // persistent: every wavefront takes work items from the launch's
-    // counter until the list is exhausted
-    for (;;) {
+    // one work item per wavefront, or (persistent grids) items taken from the
+    // launch's counter until the list is exhausted; one call site keeps the
+    // item's code inlined once
+    auto next_item = [&]() -> int {
+        if (!ticket) return (int)blockIdx.x;
         int tk0 = 0;
         if ((threadIdx.x & 63) == 0)
             tk0 = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int item = __builtin_amdgcn_readfirstlane(tk0);
-        if (item >= nitems) break;
+        return __builtin_amdgcn_readfirstlane(tk0);
+    };
+    for (int item = next_item(); item < nitems; item = next_item()) {
         run_item(item);
+        if (!ticket) break;
     }
 }
 
