@@ -502,8 +502,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--row-split', type=int, default=1, choices=(0, 1),
                     help='two-pass (a-side / b-side) digestion of the small s/p row classes')
-    ap.add_argument('--persistent', type=int, default=1, choices=(0, 1),
-                    help='persistent work-queue grids for the row and wavefront kernels')
+    ap.add_argument('--persistent', type=int, default=1, choices=(0, 1, 2),
+                    help='persistent work-queue grids for the row kernels: '
+                         '0 never, 1 for molecules from 2048 Cartesian AOs, 2 always')
     ap.add_argument('--no-mixed', action='store_true',
                     help='skip the mixed-precision leg of the c1/c2/c4 runs')
     ap.add_argument('--no-oracle', action='store_true',

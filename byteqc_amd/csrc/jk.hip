@@ -222,8 +222,10 @@ static bool prof_on() { return g_prof && !t_noalloc; }
 
 // Launch shape of the row and wave kernels: persistent grids (workgroups
 // that stay resident and pull work items from a counter) or one workgroup
-// per item (four for the row kernels).  Set by BQCset_launch_mode.
-static bool g_persistent = true;
+// per item (four for the row kernels): 0 never, 1 for launches that hold a
+// molecule from BQC_TILE_BIG_NAO Cartesian AOs up, 2 always.  Set by
+// BQCset_launch_mode.
+static int g_persistent = 1;
 
 // Workgroups of kern (nthr lanes) that fit on the whole device at once, from
 // the kernel's registers and LDS: waves per SIMD = 512 / VGPRs (granule 8, at
@@ -507,12 +509,16 @@ struct GroupT {
     double flops, nquart;
     V<int> items;     // int4 per item
     int role_b = 0;   // b-side pass of a split row class
+    int nao_max = 0;  // largest molecule (Cartesian AOs) with items here
 };
 
 template <template <class> class V>
 struct BuildPlanT {
     V<TaskDesc> tasks;
     V<size_t> perm_p_off, perm_q_off, bins_off;  // in ints
+    V<size_t> qrec_off;                          // in int4, per task
+    size_t nqrec = 0;                            // Q list records in all
+    int qmax = 0;                                // longest Q list
     V<int> ints;                                 // list orders and bins
     V<GroupT<V>> groups;
     double flops[2] = {0, 0};
@@ -520,7 +526,7 @@ struct BuildPlanT {
     GroupT<V> &group(RowKernel k, int wave, int cls, int prec) {
         for (GroupT<V> &g : groups)
             if (g.kern == k) return g;
-        groups.push_back(GroupT<V>{k, wave, cls, prec, 0, 0, {}, 0});
+        groups.push_back(GroupT<V>{k, wave, cls, prec, 0, 0, {}, 0, 0});
         return groups.back();
     }
 };
@@ -641,6 +647,9 @@ static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
     }
     bp_.perm_p_off.push_back(off_p);
     bp_.perm_q_off.push_back(off_q);
+    bp_.qrec_off.push_back(bp_.nqrec);
+    bp_.nqrec += nq;
+    if (nq > bp_.qmax) bp_.qmax = nq;
     const int cls = eval_cls(c);
     GroupT<V> *g[2] = {&bp_.group(k[0], wv[0], cls, 0), nullptr};
     g[1] = &bp_.group(k[1], wv[1], cls, 1);
@@ -648,6 +657,7 @@ static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
     for (int prec = 0; prec < 2; ++prec) {
         g[prec]->flops += nquart[prec] * c.qflops;
         g[prec]->nquart += nquart[prec];
+        if (jk.nao > g[prec]->nao_max) g[prec]->nao_max = jk.nao;
     }
     // Q end of P position p's bin and the first Q position computed in FP32
     auto q_limits = [&](int p, int &qmax, int &qsplit) {
@@ -698,8 +708,10 @@ static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
         bp_.tasks.push_back(td2);
         bp_.bins_off.push_back(bins2);
         const size_t permq2 = bp_.perm_q_off[ti];
+        const size_t qrec2 = bp_.qrec_off[ti];
         bp_.perm_p_off.push_back(bp_.ints.size());
         bp_.perm_q_off.push_back(permq2);
+        bp_.qrec_off.push_back(qrec2);  // the same Q list and records
         side_order(pbins, nbins, order2 + locs[cp_p], locs[cp_p + 1] - locs[cp_p], bp_.ints);
         GroupT<V> &g1 = bp_.group(k[0], 0, cls, 0);
         const size_t n1 = g1.items.size();
@@ -707,6 +719,7 @@ static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
         while (first >= 4 && g1.items[first - 4] == ti) first -= 4;
         GroupT<V> &g2 = bp_.group(ks[1], 0, cls, 0);
         g2.role_b = 1;
+        if (jk.nao > g2.nao_max) g2.nao_max = jk.nao;
         GroupT<V> &g1b = bp_.group(k[0], 0, cls, 0);  // group() may reallocate
         for (size_t e = first; e < n1; e += 4) {
             g2.items.push_back(ti2);
@@ -721,7 +734,7 @@ static int plan_task(BuildPlanT<V> &bp_, const TaskCtx &c, const int *bra_order,
 // Device image of a BuildPlan: [TaskDesc array][list orders][items].
 template <template <class> class V>
 struct StagedT {
-    size_t bytes = 0, off_perm = 0, off_tickets = 0;
+    size_t bytes = 0, off_perm = 0, off_tickets = 0, off_qrec = 0;
     V<size_t> off_items;  // per group
 };
 using Staged = StagedT<StdVec>;
@@ -736,6 +749,8 @@ static StagedT<V> stage_layout(const BuildPlanT<V> &p) {
     off = align16(off + p.ints.size() * sizeof(int));
     st.off_tickets = off;  // one work counter per group, zero in the image
     off = align16(off + p.groups.size() * sizeof(int));
+    st.off_qrec = off;     // Q list records, written on the device
+    off = align16(off + p.nqrec * sizeof(int4));
     for (const GroupT<V> &g : p.groups) {
         st.off_items.push_back(off);
         off = align16(off + g.items.size() * sizeof(int));
@@ -753,6 +768,7 @@ static void stage_fill(BuildPlanT<V> &p, const StagedT<V> &st, char *dev, char *
         p.tasks[t].perm_q = perm_dev + p.perm_q_off[t];
         p.tasks[t].pbins = perm_dev + p.bins_off[t];
         p.tasks[t].qbins = perm_dev + p.bins_off[t] + p.tasks[t].nbins + 1;
+        p.tasks[t].qrec = (const int *)(dev + st.off_qrec) + 4 * p.qrec_off[t];
     }
     memcpy(host, p.tasks.data(), p.tasks.size() * sizeof(TaskDesc));
     memcpy(host + st.off_perm, p.ints.data(), p.ints.size() * sizeof(int));
@@ -762,10 +778,32 @@ static void stage_fill(BuildPlanT<V> &p, const StagedT<V> &st, char *dev, char *
                p.groups[gi].items.size() * sizeof(int));
 }
 
+// Q list records of every task (task t = blockIdx.y): plan position, both
+// shells and their AO offsets per list position, so that the J/K kernels
+// read one record per step instead of a chain of three dependent loads.
+__global__ void fill_qrec(const TaskDesc *tasks, int ntask) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int t = blockIdx.y; t < ntask; t += gridDim.y) {
+        const TaskDesc &tk = tasks[t];
+        if (q >= tk.qbins[tk.nbins]) continue;
+        const int qpos = tk.perm_q[q];
+        const int gq = tk.bas_q0 + qpos;
+        const int sc = tk.pd.bra[gq], sd = tk.pd.ket[gq];
+        const unsigned c0 = (unsigned)tk.pd.ao_loc[sc], d0 = (unsigned)tk.pd.ao_loc[sd];
+        ((int4 *)tk.qrec)[q] = make_int4(qpos, sc, sd, (int)(c0 | d0 << 16));
+    }
+}
+
 template <template <class> class V>
 static int launch_groups(const BuildPlanT<V> &p, const StagedT<V> &st, char *dev,
                          hipStream_t stream) {
     const TaskDesc *tasks = (const TaskDesc *)dev;
+    if (!p.tasks.empty() && p.qmax > 0) {
+        const int nt = (int)p.tasks.size();
+        hipLaunchKernelGGL(fill_qrec, dim3((p.qmax + 255) / 256, nt < 65535 ? nt : 65535),
+                           dim3(256), 0, stream, tasks, nt);
+        if (hipGetLastError() != hipSuccess) return 1;
+    }
     for (size_t gi = 0; gi < p.groups.size(); ++gi) {
         const GroupT<V> &g = p.groups[gi];
         int nitems = (int)g.items.size() / 4;
@@ -774,7 +812,10 @@ static int launch_groups(const BuildPlanT<V> &p, const StagedT<V> &st, char *dev
         unsigned nblk = g.wave ? (unsigned)nitems : (unsigned)((nitems + 3) / 4);
         unsigned nthr = g.wave ? 64 : 256;
         int *ticket = nullptr;
-        if (g_persistent) {
+        // persistent grids pay off for large molecules (C2, C4); batches of
+        // small ones ran 5 % faster with one workgroup per item
+        if (!g.wave &&
+            (g_persistent == 2 || (g_persistent == 1 && g.nao_max >= BQC_TILE_BIG_NAO))) {
             // as many workgroups as stay resident at once; they drain the list
             const unsigned cap = resident_blocks((const void *)g.kern, (int)nthr);
             if (cap > 0 && cap < nblk) {
@@ -1238,8 +1279,8 @@ int BQCset_row_split(int cls, int on) {
 }
 
 int BQCset_launch_mode(int persistent) {
-    if (persistent != 0 && persistent != 1) return 1;
-    g_persistent = persistent == 1;
+    if (persistent < 0 || persistent > 2) return 1;
+    g_persistent = persistent;
     return 0;
 }
 

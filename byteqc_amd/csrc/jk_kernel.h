@@ -74,6 +74,10 @@ struct PairData {
 struct TaskDesc {
     PairData pd;
     const int *perm_p, *perm_q;  // list position -> plan position
+    // per Q list position {plan position, first shell, second shell,
+    // AO offset of the first | AO offset of the second << 16}, filled on
+    // the device by fill_qrec before the J/K launches of a build
+    const int *qrec;  // 4 ints per position
     const double *dm;
     double *vj, *vk;
     int nao, n_dm;

@@ -188,7 +188,8 @@ __global__ void __launch_bounds__(256)
         const gptr<int> lbra = glb(pd.bra), lket = glb(pd.ket), lao = glb(pd.ao_loc);
         const gptr<double> lbx = glb(pd.bas_x), lby = glb(pd.bas_y), lbz = glb(pd.bas_z);
         const cptr<int> ubra = cst(pd.bra), uket = cst(pd.ket), uao = cst(pd.ao_loc);
-        const cptr<int> uperm_q = cst(tk.perm_q);
+        // Q list records {plan position, first shell, second shell, AO offsets}
+        const cptr<int> uqrec = cst(tk.qrec);
         const cptr<double> ubx = cst(pd.bas_x), uby = cst(pd.bas_y), ubz = cst(pd.bas_z);
         const int gp = tk.bas_p0 + ppos;
         const int sa = pvalid ? lbra[gp] : -1;
@@ -234,16 +235,25 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int n = 0; n < NB * NC; ++n) kbc[n] = 0.0;
 
-            int gq = tk.bas_q0 + uperm_q[q0];
-            int sc = __builtin_amdgcn_readfirstlane(ubra[gq]);
+            // the records of steps [qb, qb + 64), one per lane, from one
+            // coalesced load; a step reads its own and the next step's first
+            // shell with v_readlane, so the Q walk issues no scalar loads
+            int rqx = __builtin_amdgcn_readfirstlane(uqrec[4 * q0]), rqy = __builtin_amdgcn_readfirstlane(uqrec[4 * q0 + 1]);
+            int rqz = __builtin_amdgcn_readfirstlane(uqrec[4 * q0 + 2]), rqw = __builtin_amdgcn_readfirstlane(uqrec[4 * q0 + 3]);
+            int sc = rqy;
             // D(b,c), D(a,c): loaded at the start of every run of equal c
             double dbc[NB * NC], dac[NA * NC];
             bool new_c = true;
             for (int q = q0; q < q1; ++q) {
-                const int qpos = __builtin_amdgcn_readfirstlane(uperm_q[q]);
-                gq = tk.bas_q0 + qpos;
-                const int sd = __builtin_amdgcn_readfirstlane(uket[gq]);
-                const int c0 = uao[sc], d0 = uao[sd];
+                const int qpos = rqx, sd = rqz;
+                const int c0 = rqw & 0xffff, d0 = (int)((unsigned)rqw >> 16);
+                int rnx = rqx, rny = rqy, rnz = rqz, rnw = rqw;
+                if (q + 1 < q1) {
+                    rnx = __builtin_amdgcn_readfirstlane(uqrec[4 * q + 4]);
+                    rny = __builtin_amdgcn_readfirstlane(uqrec[4 * q + 5]);
+                    rnz = __builtin_amdgcn_readfirstlane(uqrec[4 * q + 6]);
+                    rnw = __builtin_amdgcn_readfirstlane(uqrec[4 * q + 7]);
+                }
                 bool active = pvalid && q < qlim;
                 double norm = TWO_PI_52;
                 if (tk.diag) {
@@ -252,15 +262,18 @@ __global__ void __launch_bounds__(256)
                 }
                 T g[C::NF];
                 if (active) {
+                    // first Q primitive pair, marked wave-uniform so that the
+                    // primitive loop reads its data with scalar loads
+                    const int qq0 = __builtin_amdgcn_readfirstlane(tk.prim_q0 + qpos * tk.nprim_q);
                     const double rc[3] = {ubx[sc], uby[sc], ubz[sc]};
                     const double rd[3] = {ubx[sd], uby[sd], ubz[sd]};
                     if constexpr (RYS_LDS)
                         eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
-                                                     tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
+                                                     qq0, tk.nprim_q,
                                                      norm, ra, rb, rc, rd, 0, rys_tab, p_first);
                     else
                         eri_block<LA, LB, LC, LD, T>(g, pd, pp0, tk.nprim_p,
-                                                     tk.prim_q0 + qpos * tk.nprim_q, tk.nprim_q,
+                                                     qq0, tk.nprim_q,
                                                      norm, ra, rb, rc, rd, 0, nullptr, p_first);
                 } else {
 #pragma unroll
@@ -386,11 +399,7 @@ __global__ void __launch_bounds__(256)
                 }
 #undef G
                 // end of a run of equal c: flush K(a,c), K(b,c)
-                int sc_next = -1;
-                if (q + 1 < q1) {
-                    sc_next = __builtin_amdgcn_readfirstlane(
-                        ubra[tk.bas_q0 + uperm_q[q + 1]]);
-                }
+                const int sc_next = q + 1 < q1 ? rny : -1;
                 if (VK && sc_next != sc) {
                     if constexpr (ROLE != 2) {
                         constexpr int NV = NA * NC;
@@ -443,6 +452,10 @@ __global__ void __launch_bounds__(256)
                 }
                 new_c = sc_next != sc;
                 sc = sc_next;
+                rqx = rnx;
+                rqy = rny;
+                rqz = rnz;
+                rqw = rnw;
             }
             if (ROLE != 2 && VJ && pvalid) {
 #pragma unroll

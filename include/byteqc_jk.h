@@ -208,14 +208,18 @@ int BQCset_tile(int kind, int cls, int prec, int ck);
  * too many per-class settings. */
 int BQCset_row_split(int cls, int on);
 
-/* Launch shape of the row and wavefront kernels: 1 (default) persistent
- * grids -- as many workgroups as stay resident on the device at once, each
- * wavefront pulling work items from a counter in the build's work list until
- * it is exhausted; 0 one workgroup per work item (four items per row-kernel
- * workgroup).  Same quartets and sums either way.  Process-wide; not to be
- * called while another thread plans a build.  No counterpart in the
- * reference (it launches one grid per task and bin, nr_jk_driver.cu:252-280).
- * Returns 0 ok, 1 bad argument. */
+/* Launch shape of the row kernels.  Persistent grids have as many
+ * workgroups as fit on the device at once, each wavefront pulling work items
+ * from a counter in the build's work list until it is exhausted; the other
+ * shape has one workgroup per four work items.  (The wavefront kernels always
+ * take one workgroup per item: the item loop cost them registers.)
+ * persistent: 0 never; 1 (default) for launches holding a molecule of 2048
+ * or more Cartesian AOs (measured on MI355X: 64 waters cc-pVDZ 143.0 ->
+ * 136.5 ms, 128 waters cc-pVTZ 7.90 -> 7.72 s, while batches of 4-40 waters
+ * ran 5 % slower); 2 always.  Same quartets and sums in every mode.
+ * Process-wide; not to be called while another thread plans a build.  No
+ * counterpart in the reference (it launches one grid per task and bin,
+ * nr_jk_driver.cu:252-280).  Returns 0 ok, 1 bad argument. */
 int BQCset_launch_mode(int persistent);
 
 /* The tile a build of a molecule with nao Cartesian AOs planned now would

@@ -147,14 +147,15 @@ def test_persistent_and_per_item_launches_agree_on_c3_subbatch():
     lib = load_library()
     bj = BatchJK(mols)
     try:
-        assert lib.BQCset_launch_mode(1) == 0
+        assert lib.BQCset_launch_mode(2) == 0      # persistent for every molecule size
         pers = bj.build(dms)
         assert lib.BQCset_launch_mode(0) == 0
         stat = bj.build(dms)
+        assert lib.BQCset_launch_mode(2) == 0
+        bj.capture(dms)
     finally:
         lib.BQCset_launch_mode(1)
-    assert lib.BQCset_launch_mode(2) == 1
-    bj.capture(dms)
+    assert lib.BQCset_launch_mode(3) == 1
     for rep in range(3):
         x = torch.randn(200, 200, dtype=torch.float64, device='cuda')
         (x @ x).sum().item()
