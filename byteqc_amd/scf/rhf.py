@@ -12,6 +12,16 @@ Converged when |dE| < conv_tol and max|e| < conv_tol_grad.
 
 ``precision='mixed_then_fp64'`` builds J/K in mixed precision while
 |dE| >= switch_de and in FP64 afterwards (and for the final energy).
+
+Incremental builds (``incremental=True``, the default): J and K of the
+density change dD = D - D_last are added to those of D_last, as PySCF's
+direct-SCF ``get_veff`` does (vhf = vhf_last + J(dD) - K(dD)/2, pyscf
+scf/hf.py) -- the SCF loop the reference runs, since its RHF
+(cuobc/scf/hf.py:272-274) only replaces ``get_jk`` and leaves ``_eri``
+unset.  The density prescreen then drops the quartets whose Schwarz bound
+times max|dD| of their blocks is below the cutoff, so late iterations
+evaluate a small part of the tensor.  The first build, and the first
+build after a change of precision, use the whole density.
 """
 from __future__ import annotations
 
@@ -137,10 +147,11 @@ class RHF:
 
     def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=80, diis_space=8,
                  precision='fp64', switch_de=1e-5, fp32_cut=1e-7, lindep=1e-9, vhfopt=None,
-                 stream=None, init_guess='atom'):
+                 stream=None, init_guess='atom', incremental=True):
         if init_guess not in ('atom', 'hcore'):
             raise ValueError(f'unknown initial guess {init_guess!r}')
         self.init_guess = init_guess
+        self.incremental = incremental
         self.mol = mol
         self.conv_tol = conv_tol
         self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else conv_tol ** 0.5
@@ -156,7 +167,14 @@ class RHF:
         self.e_tot = None
         self.cycles = 0
         self.history = []
+        self.build_flops = []   # algorithmic FLOPs of each standalone step's build
         self.jk_builds = {'fp64': 0, 'mixed': 0}
+        # J, K of the density of the last build, its precision and density;
+        # whether the build in flight is of dD (set by next_density)
+        self._jk_last = None
+        self._jk_last_prec = None
+        self._dm_last = None
+        self._pending = None
 
     # -- set-up --------------------------------------------------------------
     def build(self):
@@ -211,19 +229,38 @@ class RHF:
 
     def _step(self):
         prec = self.next_precision()
-        j, k = hf.get_jk(self.mol, self.dm, vhfopt=self._opt, return_device=True,
-                         precision=prec, fp32_cut=self.fp32_cut, stream=self.stream)
+        j, k = hf.get_jk(self.mol, self.next_density(prec), vhfopt=self._opt,
+                         return_device=True, precision=prec, fp32_cut=self.fp32_cut,
+                         stream=self.stream)
+        self.build_flops.append(sum(hf.last_build_flops()))
         return self.absorb(j, k, prec)
 
     def next_precision(self):
         """Precision of the next J/K build ('fp64' or 'mixed')."""
         return self._jk_precision()
 
+    def next_density(self, prec):
+        """The density the next J/K build (at precision prec) must use: the
+        change since the last build when building incrementally, else the
+        current density.  absorb() must follow with that build's J, K."""
+        incr = (self.incremental and self._jk_last is not None
+                and prec == self._jk_last_prec)
+        self._pending = incr
+        return self.dm - self._dm_last if incr else self.dm
+
     def absorb(self, j, k, prec):
-        """Advance one iteration with J, K of the current density (built at
-        precision prec by any caller, e.g. a batched build); returns True
-        once converged."""
+        """Advance one iteration with J, K built (at precision prec, by any
+        caller, e.g. a batched build) from the density next_density()
+        returned -- or from the current density if it was not called;
+        returns True once converged."""
         self.jk_builds[prec] += 1
+        if self._pending:
+            j = self._jk_last[0] + j
+            k = self._jk_last[1] + k
+        self._pending = None
+        self._jk_last = (j, k)
+        self._jk_last_prec = prec
+        self._dm_last = self.dm
         f = self.H + j - 0.5 * k
         e_elec = 0.5 * float(torch.sum(self.dm * (self.H + f)))
         e_tot = e_elec + self.e_nuc

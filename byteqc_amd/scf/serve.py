@@ -46,9 +46,9 @@ from byteqc_amd.scf.batch import BatchJK
 from byteqc_amd.scf.rhf import RHF
 
 # device matrices of one SCF request besides the J/K build buffers: S, H, X,
-# D, F, MO coefficients, orbital-basis work matrices, DIIS history (F and
-# error per slot)
-_SCF_MATS = 10
+# D, F, MO coefficients, orbital-basis work matrices, J, K and D of the last
+# build (incremental builds), DIIS history (F and error per slot)
+_SCF_MATS = 13
 _JK_CART_MATS = 5   # C D C^T, J, K, two transform temporaries (Cartesian size)
 
 
@@ -232,7 +232,7 @@ class Server:
             groups[(prec, r.opt.default_nbins())].append(r)
         builds = []
         for (prec, nbins), rs in groups.items():
-            dms = [r.scf.dm if r.kind == 'scf' else r.dm for r in rs]
+            dms = [r.scf.next_density(prec) if r.kind == 'scf' else r.dm for r in rs]
             bj = BatchJK(None, vhfopts=[r.opt for r in rs], precision=prec, nbins=nbins,
                          fp32_cut=self.fp32_cut, stream=lane.stream)
             builds.append((prec, rs, bj, bj.build(dms)))
