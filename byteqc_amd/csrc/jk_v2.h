@@ -187,7 +187,6 @@ __global__ void __launch_bounds__(256)
         // per-lane (P side) and wave-uniform (Q side) views of the plan data
         const gptr<int> lbra = glb(pd.bra), lket = glb(pd.ket), lao = glb(pd.ao_loc);
         const gptr<double> lbx = glb(pd.bas_x), lby = glb(pd.bas_y), lbz = glb(pd.bas_z);
-        const cptr<int> ubra = cst(pd.bra), uket = cst(pd.ket), uao = cst(pd.ao_loc);
         // Q list records {plan position, first shell, second shell, AO offsets}
         const cptr<int> uqrec = cst(tk.qrec);
         const cptr<double> ubx = cst(pd.bas_x), uby = cst(pd.bas_y), ubz = cst(pd.bas_z);

@@ -45,6 +45,7 @@ _SIGNATURES = {
     'BQCset_tile': (c_int, [c_int, c_int, c_int, c_int]),
     'BQCget_tile': (c_int, [c_int, c_int, c_int, c_int]),
     'BQCset_launch_mode': (c_int, [c_int]),
+    'BQCset_plan_threads': (c_int, [c_int]),
     'BQCset_row_split': (c_int, [c_int, c_int]),
     'BQCplan_tasks': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double,
                               c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),

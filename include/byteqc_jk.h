@@ -222,6 +222,17 @@ int BQCset_row_split(int cls, int on);
  * nr_jk_driver.cu:252-280).  Returns 0 ok, 1 bad argument. */
 int BQCset_launch_mode(int persistent);
 
+/* Host threads that plan a list of tasks (one per 16 tasks, up to n; 0
+ * restores the default: the BQC_PLAN_THREADS environment variable, else the
+ * hardware threads up to 16).  BQCbuild_jk and BQCbuild_jk_batch plan and
+ * launch class by class (the device runs the first classes while the host
+ * plans the rest), each class's tasks as one list; BQCgraph_capture plans
+ * all tasks as one list.  Blocks of tasks are planned separately and merged
+ * in list order, so the work list is the one a single thread builds.
+ * Process-wide.  No counterpart in the reference (its host loop plans one
+ * task per call, hf.py:111-166).  Returns 0 ok, 1 if n is outside 0..256. */
+int BQCset_plan_threads(int n);
+
 /* The tile a build of a molecule with nao Cartesian AOs planned now would
  * use for (kind, cls, prec); -1 for a bad kind.  No counterpart in the
  * reference. */

@@ -194,6 +194,46 @@ def test_incremental_builds_match_full_rebuilds():
         assert incr.build_flops[-1] < 0.75 * incr.build_flops[0]
 
 
+def test_parallel_planning_matches_one_thread():
+    """A batch planned by 8 host threads (blocks of tasks merged in order)
+    gives the work list of one thread: the same FLOP count to the last digit,
+    and J/K equal to the atomics' summation-order noise, eager (planned class
+    by class) and captured in a graph (planned as one list)."""
+    torch = _torch_gpu()
+    from byteqc_amd.lib import load_library
+    from byteqc_amd.scf import hf
+    from byteqc_amd.scf.batch import BatchJK
+    from byteqc_amd.systems import water_mole, box_for_density
+    lib = load_library()
+    mols = [water_mole(n, box_for_density(n), 40 + i, 'def2-svp')
+            for i, n in enumerate([3, 4, 5, 6] * 6)]
+    rng = numpy.random.default_rng(8)
+    dms = []
+    for m in mols:
+        c = numpy.linalg.qr(rng.standard_normal((m.nao_nr(), m.nelectron // 2)))[0]
+        dms.append(torch.as_tensor(2 * c @ c.T, device='cuda'))
+    bj = BatchJK(mols)
+    out = {}
+    try:
+        for nth in (1, 8):
+            assert lib.BQCset_plan_threads(nth) == 0
+            eager = [(j.cpu().numpy(), k.cpu().numpy()) for j, k in bj.build(dms)]
+            fl = list(hf.last_build_flops())
+            bj.capture(dms)
+            graph = [(j.cpu().numpy(), k.cpu().numpy()) for j, k in bj.replay()]
+            out[nth] = (eager, graph, fl)
+    finally:
+        lib.BQCset_plan_threads(0)
+    assert out[1][2] == out[8][2]
+    for which in (0, 1):
+        d = max(numpy.abs(a - b).max() for x, y in zip(out[1][which], out[8][which])
+                for a, b in zip(x, y))
+        print(f'{"graph" if which else "eager"}: FLOPs {out[1][2]}, max diff 1 vs 8 threads '
+              f'{d:.1e}', flush=True)
+        assert d < 1e-12
+    assert lib.BQCset_plan_threads(-1) == 1
+
+
 def test_atomic_densities_hold_the_atoms_electrons():
     """Spherically averaged atomic densities: Tr(D S) is the electron count,
     D is symmetric and its natural occupations lie in [0, 2]."""
