@@ -8,7 +8,9 @@ iteration per ``step()`` call, so a request loop can interleave many SCF runs.
 Iteration: D = 2 C_occ C_occ^T; J, K = get_jk(D); F = H + J - K/2;
 E = 1/2 Tr[D (H + F)] + E_nuc; Pulay DIIS on e = X^T (F D S - S D F) X;
 F' = X^T F X diagonalised (X = S^-1/2, canonical, small eigenvalues dropped).
-Converged when |dE| < conv_tol and max|e| < conv_tol_grad.
+Converged when |dE| < conv_tol and max|e| < conv_tol_grad.  ``level_shift``
+(Eh, default 0) raises the virtual orbitals of the Fock matrix that is
+diagonalised, as PySCF's ``level_shift`` does.
 
 ``precision='mixed_then_fp64'`` builds J/K in mixed precision while
 |dE| >= switch_de and in FP64 afterwards (and for the final energy).
@@ -147,11 +149,12 @@ class RHF:
 
     def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=80, diis_space=8,
                  precision='fp64', switch_de=1e-5, fp32_cut=1e-7, lindep=1e-9, vhfopt=None,
-                 stream=None, init_guess='atom', incremental=True):
+                 stream=None, init_guess='atom', incremental=True, level_shift=0.0):
         if init_guess not in ('atom', 'hcore'):
             raise ValueError(f'unknown initial guess {init_guess!r}')
         self.init_guess = init_guess
         self.incremental = incremental
+        self.level_shift = level_shift
         self.mol = mol
         self.conv_tol = conv_tol
         self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else conv_tol ** 0.5
@@ -280,6 +283,12 @@ class RHF:
             self.converged = True
             return True
         f = self._diis(f, err)
+        if self.level_shift:
+            # PySCF's level_shift (scf/hf.py): raise the virtual space by the
+            # shift, f + shift (S - S (D/2) S); F D S - S D F and so the
+            # converged solution do not change
+            sd = self.S @ self.dm
+            f = f + self.level_shift * (self.S - 0.5 * sd @ self.S)
         self.mo_coeff = self._diag(f)
         self.dm = self._make_dm(self.mo_coeff)
         return False

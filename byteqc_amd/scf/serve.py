@@ -121,7 +121,7 @@ class Server:
 
     def __init__(self, hbm_budget=None, max_active=64, precision='mixed_then_fp64',
                  conv_tol=1e-8, switch_de=1e-5, max_cycle=80, fp32_cut=1e-7, log=None,
-                 lanes=(800, 1600), init_guess='atom'):
+                 lanes=(800, 1600), init_guess='atom', diis_space=12):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * free)
@@ -131,7 +131,8 @@ class Server:
         self._setup_stream = None
         self.max_active = max_active
         self.scf_defaults = dict(precision=precision, conv_tol=conv_tol, switch_de=switch_de,
-                                 max_cycle=max_cycle, fp32_cut=fp32_cut, init_guess=init_guess)
+                                 max_cycle=max_cycle, fp32_cut=fp32_cut, init_guess=init_guess,
+                                 diis_space=diis_space)
         self.fp32_cut = fp32_cut
         self.queue = collections.deque()
         self.active = []
@@ -154,7 +155,7 @@ class Server:
     # -- requests ----------------------------------------------------------------
     def submit(self, mol, kind='scf', dm=None, arrival=None, **scf_kwargs):
         r = Request(next(self._ids), mol, kind, dm, arrival, **scf_kwargs)
-        r.footprint = estimate_footprint(mol)
+        r.footprint = estimate_footprint(mol, self.scf_defaults['diis_space'])
         if r.footprint > self.hbm_budget:
             r.error = (f'request needs ~{r.footprint / 2**30:.2f} GiB of HBM, above the '
                        f'budget of {self.hbm_budget / 2**30:.2f} GiB')

@@ -53,7 +53,7 @@ def test_serving_matches_standalone_rhf():
     r = srv.submit(mol)
     srv.submit(_mols()[2])
     srv.drain()
-    ref = RHF(mol, precision='mixed_then_fp64').kernel()
+    ref = RHF(mol, precision='mixed_then_fp64', diis_space=srv.scf_defaults['diis_space']).kernel()
     assert abs(r.result['e_tot'] - ref) <= 1e-9
 
 
