@@ -750,8 +750,8 @@ static StagedT<V> stage_layout(const BuildPlanT<V> &p) {
     size_t off = align16(p.tasks.size() * sizeof(TaskDesc));
     st.off_perm = off;
     off = align16(off + p.ints.size() * sizeof(int));
-    st.off_tickets = off;  // one work counter per group, zero in the image
-    off = align16(off + p.groups.size() * sizeof(int));
+    st.off_tickets = off;  // work counters, BQC_TICKET_SLICES per group, zero in the image
+    off = align16(off + p.groups.size() * BQC_TICKET_SLICES * sizeof(int));
     st.off_qrec = off;     // Q list records, written on the device
     off = align16(off + p.nqrec * sizeof(int4));
     for (const GroupT<V> &g : p.groups) {
@@ -775,7 +775,7 @@ static void stage_fill(BuildPlanT<V> &p, const StagedT<V> &st, char *dev, char *
     }
     memcpy(host, p.tasks.data(), p.tasks.size() * sizeof(TaskDesc));
     memcpy(host + st.off_perm, p.ints.data(), p.ints.size() * sizeof(int));
-    memset(host + st.off_tickets, 0, p.groups.size() * sizeof(int));
+    memset(host + st.off_tickets, 0, p.groups.size() * BQC_TICKET_SLICES * sizeof(int));
     for (size_t gi = 0; gi < p.groups.size(); ++gi)
         memcpy(host + st.off_items[gi], p.groups[gi].items.data(),
                p.groups[gi].items.size() * sizeof(int));
@@ -823,7 +823,7 @@ static int launch_groups(const BuildPlanT<V> &p, const StagedT<V> &st, char *dev
             const unsigned cap = resident_blocks((const void *)g.kern, (int)nthr);
             if (cap > 0 && cap < nblk) {
                 nblk = cap;
-                ticket = (int *)(dev + st.off_tickets) + gi;
+                ticket = (int *)(dev + st.off_tickets) + gi * BQC_TICKET_SLICES;
             }
         }
         LaunchRec rec;
@@ -1369,7 +1369,7 @@ void *BQCgraph_capture(int nmol, BasisProdCache **bps, double **vj, double **vk,
         plan.flops[1] += fl[1];
         if (!err && !plan.groups.empty()) {
             // replays start from zero work counters (a kernel, not a memset)
-            const int ng = (int)plan.groups.size();
+            const int ng = (int)plan.groups.size() * BQC_TICKET_SLICES;
             hipLaunchKernelGGL(zero_ints, dim3((ng + 255) / 256), dim3(256), 0, cs,
                                (int *)(g->dev + st.off_tickets), ng);
             err = launch_groups(plan, st, g->dev, cs);

@@ -31,6 +31,10 @@
 #include "jk_kernel.h"
 
 #define BQC_V2_CK 64
+// Work counters per persistent launch: the item list is cut into this many
+// contiguous slices, one per XCD (workgroups are dispatched to the XCDs in
+// turn, so workgroup b runs on XCD b % 8)
+#define BQC_TICKET_SLICES 8
 
 // Cross-lane moves of a double through DPP (two 32-bit halves).  Lanes
 // whose source is outside the pattern, or whose row is masked off, read 0.
@@ -466,15 +470,28 @@ __global__ void __launch_bounds__(256)
             }
         }
     };
-    // one work item per wavefront, or (persistent grids) items taken from the
-    // launch's counter until the list is exhausted; one call site keeps the
-    // item's code inlined once
+    // one work item per wavefront, or (persistent grids) items taken from
+    // the slice counters until the list is exhausted: a wavefront starts on
+    // the slice of its XCD, so the items an XCD runs together are neighbours
+    // in the list (nearby shells: their density reads and J/K atomics share
+    // that XCD's L2), and moves on to the next slice when its own is empty.
+    // One call site keeps the item's code inlined once.
+    int slice = blockIdx.x % BQC_TICKET_SLICES, tries = 0;
     auto next_item = [&]() -> int {
         if (!ticket) return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-        int tk0 = 0;
-        if ((threadIdx.x & 63) == 0)
-            tk0 = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return __builtin_amdgcn_readfirstlane(tk0);
+        while (tries < BQC_TICKET_SLICES) {
+            const int lo = (int)((long long)nitems * slice / BQC_TICKET_SLICES);
+            const int hi = (int)((long long)nitems * (slice + 1) / BQC_TICKET_SLICES);
+            int tk0 = 0;
+            if ((threadIdx.x & 63) == 0)
+                tk0 = __hip_atomic_fetch_add(ticket + slice, 1, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            const int it = lo + __builtin_amdgcn_readfirstlane(tk0);
+            if (it < hi) return it;
+            slice = slice + 1 == BQC_TICKET_SLICES ? 0 : slice + 1;
+            ++tries;
+        }
+        return nitems;
     };
     for (int item = next_item(); item < nitems; item = next_item()) {
         run_item(item);
