@@ -419,6 +419,7 @@ static std::vector<std::pair<long long, int>> g_ck_override;  // (key, ck)
 // classes with 32-64 and the wave kernels with 64; for batches of small
 // molecules (4-40 H2O def2-SVP, at most ~1000 AOs) with 32, 16 and 16.
 #define BQC_TILE_BIG_NAO 2048
+#define BQC_TILE_HUGE_NAO 8192
 static bool g_ck_all_set[2] = {false, false};  // a tile set for all classes of a kind
 
 static int class_components(int cls) {
@@ -431,10 +432,14 @@ static int tile_for(int kind, int cls, int prec, int nao) {
     for (auto &kv : g_ck_override)
         if (kv.first == key) return kv.second;
     if (g_ck_all_set[kind == 1 ? 0 : 1]) return kind == 1 ? g_ck_rows : g_ck_wave;
-    const bool big = nao >= BQC_TILE_BIG_NAO;
-    if (kind == 4) return big ? 64 : 16;
-    if (class_components(cls) > 81) return big ? 64 : 16;
-    return big ? 256 : 32;
+    // three size bands, read off the autotuner on MI355X (batches of 4-40
+    // waters, 64 waters cc-pVDZ, 128 waters cc-pVTZ): longer Q ranges per
+    // item pay as the molecule grows
+    const int band = nao >= BQC_TILE_HUGE_NAO ? 2 : (nao >= BQC_TILE_BIG_NAO ? 1 : 0);
+    static const int wave[3] = {16, 32, 64}, rows_big[3] = {16, 64, 128}, rows[3] = {32, 64, 256};
+    if (kind == 4) return wave[band];
+    if (class_components(cls) > 81) return rows_big[band];
+    return rows[band];
 }
 
 // Orders of the pair lists of a plan by first shell (which = 0; for the

@@ -74,7 +74,9 @@ def tune(run, candidates=None, reps=1, nao=0):
                 raise ValueError(f'bad tile {ck}')
             agg = defaultdict(float)
             for r in _timed(run, reps):
-                if (r['kind'] & ~2) != kind:
+                # kind: 1 row / 4 wavefront kernel, + 2 FP32, + 8 b-side pass
+                # of a split class (both passes of a class share its tile)
+                if (r['kind'] & ~10) != kind:
                     continue
                 agg[(r['cls'], (r['kind'] >> 1) & 1)] += r['ms']
             for (cls, prec), ms in agg.items():

@@ -116,9 +116,9 @@ def test_autotuned_tiles_keep_results():
     from byteqc_amd.lib import load_library
     lib = load_library()
     # default tiles by molecule size: s/p rows, 108-component rows, waves
-    assert [lib.BQCget_tile(1, 1000, 0, n) for n in (500, 2624)] == [32, 256]
-    assert [lib.BQCget_tile(1, 2120, 1, n) for n in (500, 2624)] == [16, 64]
-    assert [lib.BQCget_tile(4, 2111, 0, n) for n in (500, 10368)] == [16, 64]
+    assert [lib.BQCget_tile(1, 1000, 0, n) for n in (500, 2624, 10368)] == [32, 64, 256]
+    assert [lib.BQCget_tile(1, 2120, 1, n) for n in (500, 2624, 10368)] == [16, 64, 128]
+    assert [lib.BQCget_tile(4, 2111, 0, n) for n in (500, 2624, 10368)] == [16, 32, 64]
     assert lib.BQCget_tile(2, 0, 0, 100) == -1
     lib.BQCset_tile(1, -1, 0, 128)          # a tile for all classes of a kind beats the defaults
     lib.BQCset_tile(1, 1000, 0, 16)         # and a per-class tile beats that

@@ -228,31 +228,33 @@ def test_fractional_aufbau_occupations():
 
 
 def test_default_tiles_follow_molecule_size():
-    """The tile rule is host logic in the library (no GPU call): rows of s/p
-    classes 256 Q pairs per item from 2048 Cartesian AOs up and 32 below,
-    108-component row classes 64 / 16, wave classes 64 / 16; a tile set for
-    all classes of a kind overrides the rule and a per-class tile overrides
-    both; kind 0 restores the rule."""
+    """The tile rule is host logic in the library (no GPU call): Q pairs per
+    item by Cartesian AO count, below 2048 / from 2048 / from 8192: row
+    classes up to 81 components 32 / 64 / 256, larger row classes 16 / 64 /
+    128, wave classes 16 / 32 / 64; a tile set for all classes of a kind
+    overrides the rule and a per-class tile overrides both; kind 0 restores
+    the rule."""
     from byteqc_amd.lib import library_path
     path = library_path()
     if not os.path.exists(path):
         pytest.skip('libbqcjk.so not built')
     lib = ctypes.CDLL(path)
     get = lib.BQCget_tile
-    assert [get(1, 1000, 0, n) for n in (500, 2047, 2048, 10368)] == [32, 32, 256, 256]
-    assert [get(1, 2120, 1, n) for n in (500, 2624)] == [16, 64]
-    assert [get(4, 2111, 0, n) for n in (500, 10368)] == [16, 64]
+    assert [get(1, 1000, 0, n) for n in (500, 2047, 2048, 8191, 8192, 10368)] == \
+        [32, 32, 64, 64, 256, 256]
+    assert [get(1, 2120, 1, n) for n in (500, 2624, 10368)] == [16, 64, 128]
+    assert [get(4, 2111, 0, n) for n in (500, 2624, 10368)] == [16, 32, 64]
     assert get(2, 0, 0, 100) == -1
     try:
         assert lib.BQCset_tile(1, -1, 0, 128) == 0
         assert lib.BQCset_tile(1, 1000, 0, 16) == 0
         assert get(1, 0, 0, 3000) == 128 and get(1, 2120, 1, 100) == 128
         assert get(1, 1000, 0, 3000) == 16 and get(1, 1000, 1, 3000) == 128
-        assert get(4, 2111, 0, 3000) == 64
+        assert get(4, 2111, 0, 3000) == 32
         assert lib.BQCset_tile(1, 1000, 0, 0) == 1 and lib.BQCset_tile(3, 1000, 0, 8) == 1
     finally:
         lib.BQCset_tile(0, -1, 0, 0)
-    assert get(1, 1000, 0, 3000) == 256
+    assert get(1, 1000, 0, 3000) == 64
 
 
 def test_cartesian_ao_limit_is_refused():

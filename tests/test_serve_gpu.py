@@ -61,9 +61,9 @@ def test_admission_respects_hbm_budget():
     _torch_gpu()
     from byteqc_amd.scf.serve import Server, estimate_footprint
     mols = _mols()
-    fp = [estimate_footprint(m) for m in mols]
+    fp = [estimate_footprint(m, diis_space=12) for m in mols]
     budget = max(fp) + min(fp) // 2          # never two of the larger ones at once
-    srv = Server(hbm_budget=budget)
+    srv = Server(hbm_budget=budget, diis_space=12)
     reqs = [srv.submit(m) for m in mols]
     srv.drain()
     assert all(r.error is None and r.result['converged'] for r in reqs)
