@@ -298,7 +298,12 @@ __global__ void __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>()))
                             }
                         }
                         wave_sync();
+                        // primitive pairs sorted by |e12|, negligible ones
+                        // zeroed: stop before a zero one (tested after the
+                        // work, so single-primitive pairs never wait on it)
+                        if (iq + 1 < tk.nprim_q && ue12[qq + 1] == 0.0) break;
                     }
+                    if (ip + 1 < tk.nprim_p && ue12[p + 1] == 0.0) break;
                 }
             }
             // phase C: digestion
