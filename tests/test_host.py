@@ -270,3 +270,83 @@ def test_cartesian_ao_limit_is_refused():
                            None, None, 0., None) == 4
     assert lib.GINTbuild_jk(None, None, None, None, 46341, 1, None, None, 10, 0, 0,
                             None) == 4
+
+
+def test_pair_builder_sorts_and_screens_primitive_pairs():
+    """GINTinit_basis_prod_cpu (host code): inside every shell pair the
+    primitive pairs hold the same (a12, e12, P) as a direct evaluation of
+    e12 = fac ci cj exp(-ai aj r^2 / (ai + aj)), sorted by |e12| from the
+    largest; those below 1e-20 of the largest |e12| of the cache are zero,
+    and no larger one is."""
+    from byteqc_amd.lib import library_path
+    path = library_path()
+    if not os.path.exists(path):
+        pytest.skip('libbqcjk.so not built')
+    from byteqc_amd.systems import water_mole, box_for_density
+    lib = ctypes.CDLL(path)
+    mol = water_mole(4, box_for_density(4), 3, 'def2-svp')
+    pmol, _ = hf.basis_seg_contraction(mol)
+    bas = numpy.ascontiguousarray(pmol._bas, dtype=numpy.int32)
+    atm = numpy.ascontiguousarray(pmol._atm, dtype=numpy.int32)
+    env = numpy.ascontiguousarray(pmol._env, dtype=numpy.float64)
+    nbas = len(bas)
+    # one pair type per (shell group, shell group), groups by (l, nprim)
+    key = [(int(b[1]), int(b[2])) for b in bas]
+    groups = sorted(set(key))
+    bra, ket = [], []
+    for gi, g1 in enumerate(groups):
+        for g2 in groups[:gi + 1]:
+            sh1 = [i for i in range(nbas) if key[i] == g1]
+            sh2 = [j for j in range(nbas) if key[j] == g2]
+            pr = [(i, j) for i in sh1 for j in sh2 if g1 != g2 or i >= j]
+            bra.append([p[0] for p in pr])
+            ket.append([p[1] for p in pr])
+    pair2shls = numpy.ascontiguousarray(numpy.vstack([numpy.hstack(bra), numpy.hstack(ket)]),
+                                        dtype=numpy.int32)
+    locs = numpy.append(0, numpy.cumsum([len(x) for x in bra])).astype(numpy.int32)
+    ncp = len(bra)
+    coords = numpy.zeros(3 * nbas)
+    bp = ctypes.POINTER(hf.BasisProdCache)()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.GINTinit_basis_prod_cpu(ctypes.byref(bp), ctypes.c_double(0.5), p(coords), p(pair2shls),
+                                p(locs), ctypes.c_int(ncp), p(atm), ctypes.c_int(len(atm)),
+                                p(bas), ctypes.c_int(nbas), p(env))
+    c = bp.contents
+    nprim_pairs = c.primitive_pairs_locs[ncp]
+    aexyz = numpy.ctypeslib.as_array(c.aexyz, (5 * nprim_pairs,)).reshape(5, nprim_pairs).copy()
+    fac_sp = {0: 0.282094791773878143, 1: 0.488602511902919921}
+    ref = []
+    for n in range(ncp):
+        for k in range(locs[n], locs[n + 1]):
+            i, j = pair2shls[0, k], pair2shls[1, k]
+            bi, bj = bas[i], bas[j]
+            ai, aj = env[bi[5]:bi[5] + bi[2]], env[bj[5]:bj[5] + bj[2]]
+            ci, cj = env[bi[6]:bi[6] + bi[2]], env[bj[6]:bj[6] + bj[2]]
+            ri, rj = env[atm[bi[0], 1]:atm[bi[0], 1] + 3], env[atm[bj[0], 1]:atm[bj[0], 1] + 3]
+            f = fac_sp.get(int(bi[1]), 1.0) * fac_sp.get(int(bj[1]), 1.0) * (0.5 if i == j else 1.0)
+            a12 = numpy.add.outer(ai, aj).ravel()
+            e12 = (f * numpy.outer(ci, cj) *
+                   numpy.exp(-((ri - rj) ** 2).sum() * numpy.outer(ai, aj) /
+                             numpy.add.outer(ai, aj))).ravel()
+            pxyz = [(numpy.add.outer(ai * ri[d], aj * rj[d])).ravel() / a12 for d in range(3)]
+            ref.append(numpy.vstack([a12, e12] + pxyz))
+    emax = max(numpy.abs(r[1]).max() for r in ref)
+    off, nzero = 0, 0
+    for r in ref:
+        k = r.shape[1]
+        got = aexyz[:, off:off + k]
+        order = numpy.argsort(-numpy.abs(r[1]), kind='stable')
+        want = r[:, order].copy()
+        small = numpy.abs(want[1]) < 1e-20 * emax
+        want[1, small] = 0.0
+        nzero += int(small.sum())
+        assert numpy.all(numpy.diff(numpy.abs(got[1])) <= 0)
+        # equal |e12| (the (ip, jp) and (jp, ip) pairs of a diagonal shell
+        # pair) may come in either order: compare the columns as sets
+        canon = lambda x: x[:, numpy.lexsort([numpy.round(x[i], 9) for i in (4, 3, 2, 1, 0)])]
+        # exp(-mu r^2) of arguments near -40: the two operation orders agree
+        # to ~4e-14
+        assert numpy.allclose(canon(got), canon(want), rtol=1e-12, atol=0)
+        off += k
+    assert off == nprim_pairs and nzero > 0
+    lib.GINTdel_basis_prod(ctypes.byref(bp))
