@@ -42,6 +42,30 @@ def test_one_water_matches_oracle(basis):
     assert dj < TOL and dk < TOL
 
 
+def test_primitive_pair_screening_keeps_oracle_parity():
+    """6 waters def2-SVP: the pair builder zeroes the primitive pairs below
+    1e-20 of the largest |e12| (some of them here) and the kernels skip them;
+    the oracle keeps every primitive pair.  FP64 and mixed J/K against it."""
+    torch = _torch_gpu()
+    from byteqc_amd.scf import hf
+    from byteqc_amd.systems import water_mole, box_for_density
+    from oracle import oracle as orc
+    mol = water_mole(6, box_for_density(6), 11, 'def2-svp')
+    opt = hf._VHFOpt(mol).build()
+    e12 = opt._dev['aexyz'].reshape(5, -1)[1]
+    nzero = int((e12 == 0).sum())
+    print(f'6 H2O def2-svp: {nzero} of {e12.numel()} primitive pairs screened', flush=True)
+    assert nzero > 0
+    rng = numpy.random.default_rng(12)
+    c = numpy.linalg.qr(rng.standard_normal((mol.nao_nr(), 30)))[0]
+    dm = 2 * c @ c.T
+    jref, kref = orc.get_jk(mol, dm)
+    for prec in ('fp64', 'mixed'):
+        j, k = hf.get_jk(mol, dm, vhfopt=opt, precision=prec)
+        dj, dk = _report(f'6 H2O def2-svp {prec}', j, k, jref, kref)
+        assert dj < (TOL if prec == 'fp64' else 1e-8) and dk < (TOL if prec == 'fp64' else 1e-8)
+
+
 def test_c1_matches_golden():
     """8 waters cc-pVDZ (192 AOs) against the committed oracle fixture."""
     _torch_gpu()
