@@ -85,14 +85,15 @@ constexpr bool wave_prefetch() {
 }
 
 // Resident waves per SIMD the compiler must allow (register budget 512/N).
-// For these classes a budget of 128 registers (a few spilled values) runs
-// 3-12 % faster than the 139-180 the compiler picks by itself; for others
-// of similar size it runs up to 36 % slower (measured per class on 128 H2O
-// cc-pVTZ), so the hint is per class.
+// For these classes a budget of 128 registers runs 1-4 % faster than the
+// budget the compiler picks by itself; for others it runs up to 3x slower
+// ((dd|dp) 40 -> 128 ms), and for (dp|pp) and (fp|ds) it no longer pays
+// (measured per class on 128 H2O cc-pVTZ with every wave class hinted and
+// with none), so the hint is per class.
 template <int LA, int LB, int LC, int LD>
 constexpr int wave_min_waves() {
     constexpr int c = LA * 1000 + LB * 100 + LC * 10 + LD;
-    return (c == 2111 || c == 2220 || c == 3021 || c == 3120 || c == 3210) ? 4 : 1;
+    return (c == 2220 || c == 3021 || c == 3210 || c == 3310) ? 4 : 1;
 }
 
 // Work item: (task, P position, first and end Q position); block b takes
