@@ -75,13 +75,14 @@ __device__ __forceinline__ int gidx(int n, int d) {
 // Whether the kernel loads a quartet's density elements before its integral
 // phases (hiding their latency behind phases A and B) or in the digestion
 // phase.  The early loads hold registers through phases A and B; in the
-// classes listed here that cost a resident wave per SIMD and made the
-// kernel slower on MI355X (by 18-22 %), so they load late.  Elsewhere early
-// loads gain up to 14 % (measured per class on 128 H2O cc-pVTZ).
+// classes listed here that cost resident waves and made the kernel slower on
+// MI355X ((fd|dp) 106 -> 144 ms, (fp|pp) 153 -> 170 ms), so they load late.
+// Elsewhere early loads gain up to 25 % ((dp|dp) 412 -> 328 ms; measured per
+// class on 128 H2O cc-pVTZ with every class early and every class late).
 template <int LA, int LB, int LC, int LD>
 constexpr bool wave_prefetch() {
     constexpr int c = LA * 1000 + LB * 100 + LC * 10 + LD;
-    return c != 2211 && c != 3310;
+    return c != 2211 && c != 3310 && c != 3221 && c != 3111;
 }
 
 // Resident waves per SIMD the compiler must allow (register budget 512/N).
