@@ -143,7 +143,11 @@ def measured_traffic(workload, kernel):
         return None
     with open(path) as f:
         rec = json.load(f)
-    k = rec.get('kernels', {}).get(kernel)
+    kernels = rec.get('kernels', {})
+    # the profiler names a one-pass row kernel with its role argument (0)
+    k = kernels.get(kernel)
+    if k is None and kernel.startswith('jk_rows<') and kernel.count(',') == 4:
+        k = kernels.get(kernel[:-1] + ',0>')
     return None if k is None else float(k['hbm_bytes_per_launch'])
 
 
@@ -500,8 +504,9 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=4,
                     help='c3: molecules the CPU baseline runs')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--row-split', type=int, default=1, choices=(0, 1),
-                    help='two-pass (a-side / b-side) digestion of the small s/p row classes')
+    ap.add_argument('--row-split', type=int, default=-1, choices=(-1, 0, 1),
+                    help='two-pass (a-side / b-side) digestion of the small s/p row classes: '
+                         '-1 the built-in per-class rule, 0 none, 1 every candidate')
     ap.add_argument('--persistent', type=int, default=1, choices=(0, 1, 2),
                     help='persistent work-queue grids for the row kernels: '
                          '0 never, 1 for molecules from 2048 Cartesian AOs, 2 always')

@@ -205,8 +205,11 @@ int BQCset_tile(int kind, int cls, int prec, int ck);
  * requests.  Only the small s/p classes have the two passes compiled, and
  * builds use them for molecules from 2048 Cartesian AOs up (below that the
  * doubled integral work cost more than the atomics saved: 256 molecules of
- * 4-40 waters took 7.2 s against 5.5 s per batch); default on.  Process-wide; not to be called while another thread plans a
- * build.  No counterpart in the reference.  Returns 0 ok, 1 bad argument or
+ * 4-40 waters took 7.2 s against 5.5 s per batch).  on: 0 or 1; for
+ * cls = -1 also -1, the built-in rule (the default): two passes for every
+ * candidate but (ps|ps) and (dp|ss), which measured as fast or faster in one
+ * pass on MI355X.  Process-wide; not to be called while another thread plans
+ * a build.  No counterpart in the reference.  Returns 0 ok, 1 bad argument or
  * too many per-class settings. */
 int BQCset_row_split(int cls, int on);
 
