@@ -507,9 +507,9 @@ def main():
     ap.add_argument('--row-split', type=int, default=-1, choices=(-1, 0, 1),
                     help='two-pass (a-side / b-side) digestion of the small s/p row classes: '
                          '-1 the built-in per-class rule, 0 none, 1 every candidate')
-    ap.add_argument('--persistent', type=int, default=1, choices=(0, 1, 2),
+    ap.add_argument('--persistent', type=int, default=2, choices=(0, 1, 2),
                     help='persistent work-queue grids for the row kernels: '
-                         '0 never, 1 for molecules from 2048 Cartesian AOs, 2 always')
+                         '0 never, 1 for molecules from 2048 Cartesian AOs, 2 always (default)')
     ap.add_argument('--no-mixed', action='store_true',
                     help='skip the mixed-precision leg of the c1/c2/c4 runs')
     ap.add_argument('--no-oracle', action='store_true',

@@ -223,12 +223,12 @@ static bool scratch_ok(const void *kern) {
 static thread_local bool t_noalloc = false;
 static bool prof_on() { return g_prof && !t_noalloc; }
 
-// Launch shape of the row and wave kernels: persistent grids (workgroups
-// that stay resident and pull work items from a counter) or one workgroup
-// per item (four for the row kernels): 0 never, 1 for launches that hold a
-// molecule from BQC_TILE_BIG_NAO Cartesian AOs up, 2 always.  Set by
+// Launch shape of the row kernels: persistent grids (workgroups that stay
+// resident and pull work items from per-XCD counters) or four work items per
+// workgroup: 0 never, 1 for launches that hold a molecule from
+// BQC_TILE_BIG_NAO Cartesian AOs up, 2 always (the default).  Set by
 // BQCset_launch_mode.
-static int g_persistent = 1;
+static int g_persistent = 2;
 
 // Workgroups of kern (nthr lanes) that fit on the whole device at once, from
 // the kernel's registers and LDS: waves per SIMD = 512 / VGPRs (granule 8, at
@@ -439,6 +439,12 @@ static int tile_for(int kind, int cls, int prec, int nao) {
     static const int wave[3] = {16, 32, 64}, rows_big[3] = {16, 64, 128}, rows[3] = {32, 64, 256};
     if (kind == 4) return wave[band];
     if (class_components(cls) > 81) return rows_big[band];
+    if (band == 0) {
+        // small molecules: the cheapest s/p classes like longer Q ranges,
+        // (pp|pp) shorter (256 molecules of 4-40 waters def2-SVP, ~50 ms)
+        if (cls == 0 || cls == 1000 || cls == 1100) return 64;
+        if (cls == 1111) return 16;
+    }
     return rows[band];
 }
 
@@ -825,8 +831,8 @@ static int launch_groups(const BuildPlanT<V> &p, const StagedT<V> &st, char *dev
         unsigned nblk = g.wave ? (unsigned)nitems : (unsigned)((nitems + 3) / 4);
         unsigned nthr = g.wave ? 64 : 256;
         int *ticket = nullptr;
-        // persistent grids pay off for large molecules (C2, C4); batches of
-        // small ones ran 5 % faster with one workgroup per item
+        // persistent grids: 64 waters cc-pVDZ 143.0 -> 136.5 ms, 256 small
+        // def2-SVP molecules 5.06 -> 4.69 s per batch (with per-XCD counters)
         if (!g.wave &&
             (g_persistent == 2 || (g_persistent == 1 && g.nao_max >= BQC_TILE_BIG_NAO))) {
             // as many workgroups as stay resident at once; they drain the list

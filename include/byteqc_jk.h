@@ -218,10 +218,12 @@ int BQCset_row_split(int cls, int on);
  * from a counter in the build's work list until it is exhausted; the other
  * shape has one workgroup per four work items.  (The wavefront kernels always
  * take one workgroup per item: the item loop cost them registers.)
- * persistent: 0 never; 1 (default) for launches holding a molecule of 2048
- * or more Cartesian AOs (measured on MI355X: 64 waters cc-pVDZ 143.0 ->
- * 136.5 ms, 128 waters cc-pVTZ 7.90 -> 7.72 s, while batches of 4-40 waters
- * ran 5 % slower); 2 always.  Same quartets and sums in every mode.
+ * persistent: 0 never; 1 for launches holding a molecule of 2048 or more
+ * Cartesian AOs; 2 (default) always (measured on MI355X: 64 waters cc-pVDZ
+ * 143.0 -> 136.5 ms, 128 waters cc-pVTZ 7.90 -> 7.72 s, batches of 256
+ * molecules of 4-40 waters 5.06 -> 4.69 s with the work counters split per
+ * XCD; before that split those batches ran 5 % slower persistent).  Same
+ * quartets and sums in every mode.
  * Process-wide; not to be called while another thread plans a build.  No
  * counterpart in the reference (it launches one grid per task and bin,
  * nr_jk_driver.cu:252-280).  Returns 0 ok, 1 bad argument. */

@@ -116,7 +116,7 @@ def test_autotuned_tiles_keep_results():
     from byteqc_amd.lib import load_library
     lib = load_library()
     # default tiles by molecule size: s/p rows, 108-component rows, waves
-    assert [lib.BQCget_tile(1, 1000, 0, n) for n in (500, 2624, 10368)] == [32, 64, 256]
+    assert [lib.BQCget_tile(1, 1010, 0, n) for n in (500, 2624, 10368)] == [32, 64, 256]
     assert [lib.BQCget_tile(1, 2120, 1, n) for n in (500, 2624, 10368)] == [16, 64, 128]
     assert [lib.BQCget_tile(4, 2111, 0, n) for n in (500, 2624, 10368)] == [16, 32, 64]
     assert lib.BQCget_tile(2, 0, 0, 100) == -1
@@ -154,7 +154,7 @@ def test_persistent_and_per_item_launches_agree_on_c3_subbatch():
         assert lib.BQCset_launch_mode(2) == 0
         bj.capture(dms)
     finally:
-        lib.BQCset_launch_mode(1)
+        lib.BQCset_launch_mode(2)             # the default
     assert lib.BQCset_launch_mode(3) == 1
     for rep in range(3):
         x = torch.randn(200, 200, dtype=torch.float64, device='cuda')
