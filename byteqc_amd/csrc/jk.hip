@@ -435,16 +435,16 @@ static int tile_for(int kind, int cls, int prec, int nao) {
     // three size bands, read off the autotuner on MI355X (batches of 4-40
     // waters, 64 waters cc-pVDZ, 128 waters cc-pVTZ): longer Q ranges per
     // item pay as the molecule grows
+    // (batches of small molecules with persistent grids: longer ranges than
+    // mid-size molecules, except (ss|ss) and (pp|pp): 256 molecules of 4-40
+    // waters def2-SVP 4.65 -> 4.52 s per batch)
     const int band = nao >= BQC_TILE_HUGE_NAO ? 2 : (nao >= BQC_TILE_BIG_NAO ? 1 : 0);
-    static const int wave[3] = {16, 32, 64}, rows_big[3] = {16, 64, 128}, rows[3] = {32, 64, 256};
+    static const int wave[3] = {32, 32, 64}, rows_big[3] = {128, 64, 128},
+                     rows[3] = {128, 64, 256};
     if (kind == 4) return wave[band];
     if (class_components(cls) > 81) return rows_big[band];
-    if (band == 0) {
-        // small molecules: the cheapest s/p classes like longer Q ranges,
-        // (pp|pp) shorter (256 molecules of 4-40 waters def2-SVP, ~50 ms)
-        if (cls == 0 || cls == 1000 || cls == 1100) return 64;
-        if (cls == 1111) return 16;
-    }
+    if (band == 0 && cls == 0) return 32;
+    if (band == 0 && cls == 1111) return 16;
     return rows[band];
 }
 

@@ -186,10 +186,11 @@ void BQCgraph_destroy(void *graph);
  * precision prec (0 FP64, 1 FP32), or for all classes of the kind (cls < 0);
  * kind 0 clears these settings and restores the default tiles, which depend
  * on the molecule's Cartesian AO count (read off the autotuner on MI355X:
- * rows / rows of classes above 81 components / waves 32 / 16 / 16 below
- * 2048 AOs, 64 / 64 / 32 from 2048, 256 / 128 / 64 from 8192; 64 waters
- * cc-pVDZ, 2624 AOs, ran 6 ms faster on 64 than on 256 Q pairs per row
- * item).  Precedence when a build is planned: per-class setting,
+ * rows / rows of classes above 81 components / waves 128 / 128 / 32 below
+ * 2048 AOs (32 for (ss|ss), 16 for (pp|pp)), 64 / 64 / 32 from 2048,
+ * 256 / 128 / 64 from 8192; 64 waters cc-pVDZ, 2624 AOs, ran 6 ms faster on
+ * 64 than on 256 Q pairs per row item).  Precedence when a build is
+ * planned: per-class setting,
  * all-class setting, default.  Process-wide; used by the tile autotuner;
  * not to be called while another thread plans a build.
  * No counterpart in the reference (its launch shapes are fixed per class,

@@ -106,7 +106,7 @@ def test_autotuned_tiles_keep_results():
         for (kind, cls, prec), rec in table.items():
             # the tuned tile is installed; reset() brings back the default one
             assert autotune.current_tile(kind, cls, prec, 100) == rec['ck']
-            assert rec['builtin'] in (16, 32, 64, 256)
+            assert rec['builtin'] in (16, 32, 64, 128, 256)
         got = bj.build(dms)
         for a, b in zip(base, got):
             assert _maxdiff(a[0].cpu(), b[0].cpu()) < 1e-11
@@ -116,9 +116,9 @@ def test_autotuned_tiles_keep_results():
     from byteqc_amd.lib import load_library
     lib = load_library()
     # default tiles by molecule size: s/p rows, 108-component rows, waves
-    assert [lib.BQCget_tile(1, 1010, 0, n) for n in (500, 2624, 10368)] == [32, 64, 256]
-    assert [lib.BQCget_tile(1, 2120, 1, n) for n in (500, 2624, 10368)] == [16, 64, 128]
-    assert [lib.BQCget_tile(4, 2111, 0, n) for n in (500, 2624, 10368)] == [16, 32, 64]
+    assert [lib.BQCget_tile(1, 1010, 0, n) for n in (500, 2624, 10368)] == [128, 64, 256]
+    assert [lib.BQCget_tile(1, 2120, 1, n) for n in (500, 2624, 10368)] == [128, 64, 128]
+    assert [lib.BQCget_tile(4, 2111, 0, n) for n in (500, 2624, 10368)] == [32, 32, 64]
     assert lib.BQCget_tile(2, 0, 0, 100) == -1
     lib.BQCset_tile(1, -1, 0, 128)          # a tile for all classes of a kind beats the defaults
     lib.BQCset_tile(1, 1000, 0, 16)         # and a per-class tile beats that

@@ -230,10 +230,10 @@ def test_fractional_aufbau_occupations():
 def test_default_tiles_follow_molecule_size():
     """The tile rule is host logic in the library (no GPU call): Q pairs per
     item by Cartesian AO count, below 2048 / from 2048 / from 8192: row
-    classes up to 81 components 32 / 64 / 256 (below 2048: 64 for (ss|ss),
-    (ps|ss), (pp|ss), 16 for (pp|pp)), larger row classes 16 / 64 / 128, wave
-    classes 16 / 32 / 64; a tile set for all classes of a kind overrides the
-    rule and a per-class tile overrides both; kind 0 restores the rule."""
+    classes up to 81 components 128 / 64 / 256 (below 2048: 32 for (ss|ss),
+    16 for (pp|pp)), larger row classes 128 / 64 / 128, wave classes 32 /
+    32 / 64; a tile set for all classes of a kind overrides the rule and a
+    per-class tile overrides both; kind 0 restores the rule."""
     from byteqc_amd.lib import library_path
     path = library_path()
     if not os.path.exists(path):
@@ -241,11 +241,11 @@ def test_default_tiles_follow_molecule_size():
     lib = ctypes.CDLL(path)
     get = lib.BQCget_tile
     assert [get(1, 1000, 0, n) for n in (500, 2047, 2048, 8191, 8192, 10368)] == \
-        [64, 64, 64, 64, 256, 256]
-    assert [get(1, 1010, 0, n) for n in (500, 2624, 10368)] == [32, 64, 256]
+        [128, 128, 64, 64, 256, 256]
+    assert [get(1, 0, 0, n) for n in (500, 2624, 10368)] == [32, 64, 256]
     assert [get(1, 1111, 0, n) for n in (500, 2624, 10368)] == [16, 64, 256]
-    assert [get(1, 2120, 1, n) for n in (500, 2624, 10368)] == [16, 64, 128]
-    assert [get(4, 2111, 0, n) for n in (500, 2624, 10368)] == [16, 32, 64]
+    assert [get(1, 2120, 1, n) for n in (500, 2624, 10368)] == [128, 64, 128]
+    assert [get(4, 2111, 0, n) for n in (500, 2624, 10368)] == [32, 32, 64]
     assert get(2, 0, 0, 100) == -1
     try:
         assert lib.BQCset_tile(1, -1, 0, 128) == 0
