@@ -1037,7 +1037,12 @@ static int plan_list(BuildPlan &plan, const MolArgs *m, const TaskRef *list, siz
     std::vector<std::vector<Deferred>> part_def(nblk);
     std::vector<int> part_err(nblk, 0);
     std::atomic<size_t> next(0);
+    // planning may query kernel attributes (scratch and LDS limits): the
+    // workers use the calling thread's device, not a new thread's device 0
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     auto work = [&]() {
+        (void)hipSetDevice(dev);
         for (size_t b; (b = next.fetch_add(1)) < nblk;) {
             const size_t t0 = b * bsz, t1 = std::min(n, t0 + bsz);
             part_err[b] = plan_refs(part[b], m, list + t0, t1 - t0, nbins, precision, fp32_cut,
