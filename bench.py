@@ -421,6 +421,8 @@ def run_batch(args, torch):
     }
     if tuned:
         out['autotune'] = tuned
+    if args.copy_probe:
+        copy_probe(torch)
     if not args.no_cpu:
         ns = args.cpu_sample
         sample_flops = 0.0
