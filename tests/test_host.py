@@ -352,3 +352,28 @@ def test_pair_builder_sorts_and_screens_primitive_pairs():
         off += k
     assert off == nprim_pairs and nzero > 0
     lib.GINTdel_basis_prod(ctypes.byref(bp))
+
+
+def test_host_controls_check_their_arguments():
+    """Process-wide controls of the library (host code, no GPU call): the
+    split rule takes 0/1 per class and -1/0/1 for all classes, the planning
+    threads 0..256, the launch mode 0..2; each returns 1 on a bad value and
+    leaves the setting alone."""
+    from byteqc_amd.lib import library_path
+    path = library_path()
+    if not os.path.exists(path):
+        pytest.skip('libbqcjk.so not built')
+    lib = ctypes.CDLL(path)
+    try:
+        assert lib.BQCset_row_split(1000, 2) == 1 and lib.BQCset_row_split(1000, -1) == 1
+        assert lib.BQCset_row_split(-1, 2) == 1
+        assert lib.BQCset_row_split(1000, 0) == 0 and lib.BQCset_row_split(-1, 1) == 0
+        assert lib.BQCset_row_split(-1, -1) == 0
+        assert lib.BQCset_plan_threads(-1) == 1 and lib.BQCset_plan_threads(257) == 1
+        assert lib.BQCset_plan_threads(4) == 0 and lib.BQCset_plan_threads(0) == 0
+        assert lib.BQCset_launch_mode(3) == 1 and lib.BQCset_launch_mode(-1) == 1
+        assert lib.BQCset_launch_mode(1) == 0
+    finally:
+        lib.BQCset_row_split(-1, -1)
+        lib.BQCset_plan_threads(0)
+        lib.BQCset_launch_mode(2)
