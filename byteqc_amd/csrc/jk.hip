@@ -983,6 +983,7 @@ static int plan_threads(size_t ntask) {
         const char *e = getenv("BQC_PLAN_THREADS");
         int n = e ? atoi(e) : (int)std::thread::hardware_concurrency();
         if (!e && n > 16) n = 16;
+        if (n > 256) n = 256;
         return n < 1 ? 1 : n;
     }();
     const int set = g_plan_threads.load();
