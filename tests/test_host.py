@@ -274,6 +274,20 @@ def test_cartesian_ao_limit_is_refused():
                             None) == 4
 
 
+def test_build_entries_refuse_bad_arguments_before_the_device():
+    """BQCbuild_jk checks its arguments before planning or any device call:
+    a precision other than 0 (FP64) or 1 (mixed) returns 1, more than 4096
+    Schwarz bins returns 3; an empty task list returns 0 without work."""
+    from byteqc_amd.lib import load_library
+    lib = load_library()
+    args = lambda nao, ntask, nbins, prec: (None, None, None, None, nao, 1, ntask, None, None,
+                                            nbins, None, prec, None, None, 0., None)
+    assert lib.BQCbuild_jk(*args(100, 0, 10, 2)) == 1
+    assert lib.BQCbuild_jk(*args(100, 0, 10, -1)) == 1
+    assert lib.BQCbuild_jk(*args(100, 0, 4097, 0)) == 3
+    assert lib.BQCbuild_jk(*args(100, 0, 10, 0)) == 0
+
+
 def test_pair_builder_sorts_and_screens_primitive_pairs():
     """GINTinit_basis_prod_cpu (host code): inside every shell pair the
     primitive pairs hold the same (a12, e12, P) as a direct evaluation of
